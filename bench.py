@@ -1,0 +1,118 @@
+"""Benchmark of the sieve hot path on one MI355X.
+
+Workload (default): plain factorial + compositorial sieve, -! -c,
+n in [101, 1,000,000), p from 1e12 (the 2e9-wide p-range of the headline
+configuration, processed one p-segment at a time).  One "step" is one
+p-segment of about 2^22 candidates run through generate -> setup -> walk ->
+check, with its factor records drained; inputs are generated on the device.
+
+Prints ONE JSON line.  `value` is p*n residue updates per second over the
+timed steps (updates = candidates x multiplier steps, F and C walks counted
+separately).  `roofline` prices the updates at 19 INT32 multiplies each (the
+reference's 64-bit Montgomery product; see the roofline section of the
+design notes) against the measured INT32 multiply lane-op rate of the device.
+`cpu_baseline` times the CPU oracle (OpenMP, all host cores) on a bounded
+p-sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+W_MUL = 19  # INT32 multiply instructions per update in the reference formulation
+
+
+def composites_in(nmin, nmax):
+    import numpy as np
+    sieve = np.ones(nmax, dtype=bool)
+    sieve[:2] = False
+    for i in range(2, int(nmax ** 0.5) + 1):
+        if sieve[i]:
+            sieve[i * i::i] = False
+    return int((~sieve[nmin:nmax]).sum())
+
+
+def cpu_baseline(modes, p0, width, nmin, nmax, steps_per_cand):
+    import oracle as O
+    O.build()
+    t = time.perf_counter()
+    r = O.run(modes, p0, p0 + width, nmin, nmax)
+    dt = time.perf_counter() - t
+    upd = r.primecount * steps_per_cand
+    return {"value": upd / dt, "unit": "updates/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"-! -c p in [{p0}, {p0 + width}) ({r.primecount} candidates), n in [{nmin}, {nmax}); "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--seg-candidates", type=int, default=1 << 22)
+    ap.add_argument("--ppl", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-width", type=int, default=150000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding over p is "
+                                   "described in DESIGN.md but not built", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import pfcsieve_amd as P
+    modes, nmin, nmax = P.MODE_FACTORIAL | P.MODE_COMPOSITORIAL, 101, 1_000_000
+    pmin = 10**12
+    ncomp = composites_in(nmin, nmax)
+    steps_per_cand = (nmax - nmin) + ncomp
+
+    dev = P.Device(0)
+    rates = dev.probe_mul_rates()
+    p_mul = max(rates.values())
+    dev.set_tuning(seg_candidates=args.seg_candidates, primes_per_lane=args.ppl, block_size=args.block)
+    total = args.steps + args.warmup
+    # p-range: the headline 2e9 window, or wider if more segments are asked for
+    job = P.Job(dev, modes, pmin, pmin + 2 * 10**9 * max(1, (total + 16) // 17), nmin, nmax)
+    for _ in range(args.warmup):
+        job.run(1)
+    s0 = job.stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        job.run(1)
+    t1 = time.perf_counter()
+    s1 = job.stats()
+    cand = s1.primecount - s0.primecount
+    updates = cand * steps_per_cand
+    wall = t1 - t0
+    dev_ms = s1.device_ms - s0.device_ms
+    ups = updates / wall
+    job.destroy()
+
+    achieved = ups * W_MUL / 1e12
+    line = {
+        "metric": "p*n residue updates/sec on 1 MI355X (plain one-range-per-launch form)",
+        "value": ups, "unit": "updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64 (exact modular integer)", "data": "synthetic (device-generated primes)",
+        "config": {"workload": "-! -c, n in [101, 1e6), p from 1e12 (headline config, per p-segment)",
+                   "candidates_timed": cand, "steps_per_candidate": steps_per_cand,
+                   "seg_candidates": args.seg_candidates, "tuning": dev.tuning(), "device_ms": dev_ms},
+        "roofline": {"bound": "valu-int32-mul", "achieved": achieved, "peak": p_mul / 1e12,
+                     "unit": "Tops/s", "frac": achieved / (p_mul / 1e12), "traffic": None,
+                     "note": f"{W_MUL} INT32 multiplies per update; peak = measured lane-op rate "
+                             f"(mad_u64_u32 {rates['mad_u64_u32']:.3e}, mul_lo {rates['mul_lo_u32']:.3e}, "
+                             f"mul_hi {rates['mul_hi_u32']:.3e})"},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(modes, pmin, args.cpu_width, nmin, nmax, steps_per_cand)
+        line["cpu_baseline"]["gpu_over_cpu"] = ups / line["cpu_baseline"]["value"]
+    dev.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,221 @@
+/*
+ * pfc.h -- C ABI of the MI355X (gfx950) PFCSieve sieve library, libpfcsieve.so.
+ *
+ * The library finds 64-bit prime factors p of n!+-1, n#+-1 and n!/n#+-1 for
+ * n in [nmin, nmax) and p in [pmin, pmax), with the reference's candidate set,
+ * factor records and BOINC-quorum checksum, bit for bit.
+ *
+ * It replaces the reference's OpenCL boundary (simpleCL.h:65-119, which
+ * cl_sieve.cpp drives) and the device half of its search driver
+ * (cl_sieve.cpp:1328-1761).  Plain pointers and sizes only; every call returns
+ * an int status (PFC_OK == 0) and never exits the process: the reference's
+ * exit(EXIT_FAILURE) points (simpleCL.c:347, cl_sieve.cpp:462-486, 1061-1066)
+ * become status codes, with a message in pfc_last_error().
+ *
+ * Threading: a pfc_device may be shared by threads; each pfc_job owns one HIP
+ * stream and must be driven by one thread at a time.
+ */
+#ifndef PFC_H
+#define PFC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+#define PFC_OK 0
+#define PFC_ERR_INVALID (-1)         /* bad argument or mode set (cl_sieve.cpp:600-641) */
+#define PFC_ERR_HIP (-2)             /* HIP runtime failure (simpleCL.c:347 analogue) */
+#define PFC_ERR_SIEVE (-3)           /* primorial nmax is prime: "CPU sieve failure" (cl_sieve.cpp:1061-1066) */
+#define PFC_ERR_FACTOR_OVERFLOW (-4) /* factor records exceeded capacity (cl_sieve.cpp:484-487) */
+#define PFC_ERR_VALIDATION (-5)      /* device self-check failed (check.cl:62-73, cl_sieve.cpp:476-480) */
+#define PFC_ERR_VERIFY (-6)          /* CPU factor verification failed (cl_sieve.cpp:509-529) */
+#define PFC_ERR_NOMEM (-7)           /* host or device allocation failed */
+#define PFC_ERR_IO (-8)              /* factors.txt / checkpoint file I/O */
+#define PFC_ERR_CAPACITY (-9)        /* a size exceeds a documented limit */
+
+/* Mode bits: -! -# -c (main.cpp:84-100).  Only F|C may combine (cl_sieve.cpp:600-617). */
+#define PFC_MODE_FACTORIAL 1u
+#define PFC_MODE_PRIMORIAL 2u
+#define PFC_MODE_COMPOSITORIAL 4u
+
+/* Factor record types (cl_sieve.h:4-6). */
+#define PFC_FACTORIAL 0
+#define PFC_PRIMORIAL 1
+#define PFC_COMPOSITORIAL 2
+
+/* One factor record, 16 bytes, the reference's `factor` (cl_sieve.h:8-12,
+ * iterate.cl:17-21): nc = -n means p | X(n)-1, nc = +n means p | X(n)+1. */
+typedef struct pfc_factor {
+    uint64_t p;
+    int32_t nc;
+    int32_t type;
+} pfc_factor;
+
+/* A sieve job: the reference's workStatus search fields (cl_sieve.h:14-18). */
+typedef struct pfc_job_desc {
+    uint32_t modes;            /* PFC_MODE_* bits */
+    uint32_t nmin, nmax;       /* 101 <= nmin < nmax < 2^31 (main.cpp:66-72) */
+    uint64_t pmin, pmax;       /* 3 <= pmin < pmax <= 2^64-1 (main.cpp:58-64) */
+    uint64_t factor_capacity;  /* 0: 1,000,000, or 30,000,000 when pmin < 2^32-1
+                                  (main.cpp:215, cl_sieve.cpp:645-647) */
+} pfc_job_desc;
+
+/* Launch shapes; the counterpart of the reference's profile-driven
+ * range/sstep/nstep (cl_sieve.cpp:655-769, 1642-1694).  Zero fields mean
+ * "library default / autotuned". */
+typedef struct pfc_tuning {
+    uint32_t primes_per_lane;  /* independent residue chains per lane: 1, 2, 4 */
+    uint32_t block_size;       /* walk workgroup size: 64, 128, 256 */
+    uint32_t walk_chunk;       /* multiplier steps per walk launch */
+    uint32_t seg_candidates;   /* target candidates per p-segment */
+} pfc_tuning;
+
+/* Host-side summary of a finished job (cl_sieve.cpp:1748). */
+typedef struct pfc_job_stats {
+    uint64_t dev_checksum;     /* sum of the check terms (check.cl:81-91), wrapping */
+    uint64_t primecount;       /* candidates tested (g_sum[0], cl_sieve.cpp:456) */
+    uint64_t raw_factors;      /* factor records before the isPrime filter */
+    uint64_t segments;         /* p-segments processed */
+    double device_ms;          /* stream time of the whole job */
+} pfc_job_stats;
+
+typedef struct pfc_device pfc_device;
+typedef struct pfc_job pfc_job;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* pfc_last_error(void);
+
+/* ------------------------------------------------------------------ device */
+/* Replaces sclGetCLSoftware/clCreateContext/queue setup (main.cpp:252-330,
+ * simpleCL.c:631-668): one HIP device, kernels compiled ahead of time. */
+int pfc_device_open(int ordinal, pfc_device** out);
+int pfc_device_close(pfc_device* dev);
+int pfc_device_info(const pfc_device* dev, char* name, size_t name_cap, int* compute_units,
+                    uint64_t* hbm_bytes);
+int pfc_set_tuning(pfc_device* dev, const pfc_tuning* t);
+int pfc_get_tuning(const pfc_device* dev, pfc_tuning* t);
+
+/* ------------------------------------------------------------------ jobs */
+/* Validates like setupSearch (cl_sieve.cpp:596-665) and reports the HBM bytes
+ * a job will hold (prime state + tables + factor buffer), for admission. */
+int pfc_job_footprint(const pfc_device* dev, const pfc_job_desc* desc, uint64_t* bytes);
+/* Builds the host tables (cl_sieve.cpp:842-1326) and allocates device state. */
+int pfc_job_create(pfc_device* dev, const pfc_job_desc* desc, pfc_job** out);
+/* Runs p-segments [p, min(p + span, pmax)) until pmax or until `max_segments`
+ * segments (0 = all) have run: generate -> setup -> walk -> check per segment,
+ * the reference's main loop (cl_sieve.cpp:1563-1726).  Blocking. */
+int pfc_job_run(pfc_job* job, uint64_t max_segments);
+/* Next p to be sieved (== pmax when done); the checkpoint position (workStatus.p). */
+int pfc_job_position(const pfc_job* job, uint64_t* p);
+/* Resume position and accumulated device sums, from a checkpoint
+ * (cl_sieve.cpp:1416-1431).  Only valid before the first pfc_job_run. */
+int pfc_job_seek(pfc_job* job, uint64_t p, uint64_t dev_checksum, uint64_t primecount);
+int pfc_job_stats_get(const pfc_job* job, pfc_job_stats* st);
+/* Raw factor records collected so far, sorted by (p, |n|, type)
+ * (cl_sieve.cpp:433-447, 496-501).  With out == NULL only *n is set. */
+int pfc_job_factors(const pfc_job* job, pfc_factor* out, size_t cap, size_t* n);
+/* Drops the collected factor records (after they were harvested to disk). */
+int pfc_job_clear_factors(pfc_job* job);
+int pfc_job_destroy(pfc_job* job);
+
+/* Runs several jobs as one batch: their segments are packed into shared
+ * launches of a persistent walk kernel fed by a device work queue
+ * (no counterpart in the reference, which runs one range per launch). */
+int pfc_batch_run(pfc_job** jobs, size_t njobs);
+
+/* ------------------------------------------------------------------ harvest */
+/* getResults' host half (cl_sieve.cpp:496-593): sorts `f` by (p, |n|, type),
+ * verifies every record on the CPU with `verify_threads` OpenMP threads
+ * (verifyprime.cpp:149-218), keeps records whose p passes the 7-base
+ * Miller-Rabin isPrime (verifyprime.cpp:109-145), adds n + c of each kept
+ * record to *checksum, compacts kept records to the front of `f` and returns
+ * their number in *nkept.  verify_threads == 0 skips the CPU verification. */
+int pfc_harvest(pfc_factor* f, size_t n, int verify_threads, uint64_t* checksum, size_t* nkept);
+/* factors.txt line for one kept record: "%llu | %u!%+d", "%llu | %u#%+d" or
+ * "%llu | %u!/#%+d" plus '\n' (cl_sieve.cpp:565-577).  Returns the length. */
+int pfc_format_factor(const pfc_factor* f, char* buf, size_t cap);
+/* Deterministic 7-base Miller-Rabin (verifyprime.cpp:109-145). */
+int pfc_is_prime(uint64_t p);
+/* Slow CPU check that p divides X(n)+c (verifyprime.cpp:149-218). */
+int pfc_verify_factor(uint64_t p, uint32_t n, int32_t c, int32_t type);
+/* The reference's number grammar: K/M/G/T/P, eN, k/m/g/t/p, bN suffixes
+ * (putil.c:34-116).  Returns 0, -1 (cannot parse) or -2 (out of range). */
+int pfc_parse_uint64(const char* s, uint64_t lo, uint64_t hi, uint64_t* out);
+
+/* ------------------------------------------------------------------ stages */
+/* Stage-level entry points over caller-owned device memory, on a caller HIP
+ * stream (hipStream_t passed as void*; NULL = default stream).  Used by tests
+ * and by callers that build their own pipelines.  All pointers are device
+ * pointers unless stated.
+ *
+ * Control block, 32 bytes, zeroed by the caller before a segment:
+ *   u32 ncand, nfactor, flags, pad; u64 checksum, primecount. */
+typedef struct pfc_ctrl {
+    uint32_t ncand;      /* candidates appended (g_primecount[0]) */
+    uint32_t nfactor;    /* factor records appended (g_primecount[2]); may exceed capacity */
+    uint32_t flags;      /* bit0 candidate overflow, bit1 validation failure */
+    uint32_t pad;
+    uint64_t checksum;   /* sum of check terms (wrapping) */
+    uint64_t primecount; /* sum of ncand over checked segments */
+} pfc_ctrl;
+
+/* (a) Candidates in [lo, hi): primes below 114 plus wheel-30 numbers with no
+ * factor 7..113 that pass the base-2 strong PRP (getsegprimes.cl:464-526,
+ * addsmallprimes.cl:21-40).  Appends to d_p[0..cap) in no particular order;
+ * ctrl->ncand counts all, flag bit0 when more than cap. */
+int pfc_stage_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pfc_ctrl* d_ctrl,
+                       void* stream);
+
+/* Host-built start-residue tables (replacing cl_sieve.cpp:842-1326): chunks
+ * are products of primes below 2^32, grouped by exponent bit, high bit first. */
+typedef struct pfc_setup_table {
+    const uint32_t* chunks;    /* device */
+    const uint32_t* groups;    /* device: nbits+1 offsets into chunks, group b = bit b */
+    uint32_t nbits;
+} pfc_setup_table;
+
+/* (b) Per candidate: reciprocal, Montgomery one, and start residues
+ * Mont(X(nmin-1)) for table A (F, P or C) and table B (C in F+C mode)
+ * (setup.cl:49-288).  Residues are stored shifted left by clz64(p). */
+int pfc_stage_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uint64_t* d_one,
+                    uint64_t* d_res_a, uint64_t* d_res_b, const pfc_setup_table* a,
+                    const pfc_setup_table* b, uint32_t cap, void* stream);
+
+/* (c) Residue walk over multiplier steps [j0, j1) (iterate.cl:59-198):
+ *   modes F:    multiplier n = j                          (res_a)
+ *   modes C:    n = j when j is composite                 (res_a)
+ *   modes F|C:  n = j for res_a; composite j also res_b   (res_b)
+ *   modes P:    multiplier n = d_list[j]                  (res_a)
+ * Composite flags: bit (n - nbase) of d_composite (u32 words).
+ * Records (p, -+n, type) when a residue becomes +-1, up to fcap records. */
+int pfc_stage_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list,
+                   const uint32_t* d_composite, uint32_t nbase, const uint64_t* d_p,
+                   const uint32_t* d_v, const uint64_t* d_one, uint64_t* d_res_a,
+                   uint64_t* d_res_b, pfc_ctrl* d_ctrl, pfc_factor* d_fac, uint32_t fcap,
+                   uint32_t cap, void* stream);
+
+/* (c) Checksum fold (check.cl:45-204): adds every candidate's term to
+ * ctrl->checksum and ncand to ctrl->primecount; validates the reciprocals. */
+int pfc_stage_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint32_t* d_v,
+                    const uint64_t* d_one, const uint64_t* d_res_a, const uint64_t* d_res_b,
+                    pfc_ctrl* d_ctrl, uint32_t cap, void* stream);
+
+/* Builds the setup table for X(N) on the host: kind 0 = N!, 1 = N#, 2 = N!/N#.
+ * Writes up to chunk_cap chunks and nbits+1 group offsets (group_cap >= 33). */
+int pfc_build_setup_table(uint32_t kind, uint32_t N, uint32_t* chunks, size_t chunk_cap,
+                          size_t* nchunks, uint32_t* groups, size_t group_cap, uint32_t* nbits);
+
+/* ------------------------------------------------------------------ probe */
+/* Measures the lane-op rate of independent v_mad_u64_u32 / v_mul_lo_u32 /
+ * v_mul_hi_u32 streams on all CUs (the INT32 multiply roofline).  rates[3]
+ * receives lane-ops per second for {mad_u64_u32, mul_lo_u32, mul_hi_u32}. */
+int pfc_probe_mul_rates(pfc_device* dev, double rates[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PFC_H */

@@ -1,0 +1,170 @@
+// pfc_arith.h -- exact modular arithmetic for 64-bit odd moduli p, shared by
+// the gfx950 kernels and the host (the same source compiles for both, so the
+// host build can check the device arithmetic against big integers).
+//
+// Representation ("shifted"): for a prime p, s = clz64(p) and d = p << s, so
+// d has its top bit set.  A residue x mod p is kept as x' = x << s, i.e. as a
+// residue mod d that is a multiple of 2^s.  Multiplying x' by a plain integer
+// n and reducing mod d gives (x*n mod p) << s, so the shift is preserved.
+//
+// Reduction is the 3-by-2 division with a precomputed reciprocal
+//   v = floor((2^96 - 1) / d) - 2^32
+// (N. Moller and T. Granlund, "Improved division by invariant integers",
+// IEEE Trans. Computers 60(2), 2011, Algorithm 5).  A 64x32-bit modular
+// product costs four 32x32->64 multiplies plus one 32-bit low multiply, which
+// maps onto v_mad_u64_u32 / v_mul_lo_u32 on CDNA.
+//
+// The reference (kernels/iterate.cl:44-57) multiplies 64-bit Montgomery forms
+// with R = 2^64.  Its checksum sums Montgomery forms X*R mod p, so residues
+// here carry the same factor R: a walk starts from (X0*R mod p) << s and is
+// multiplied by plain n, which keeps the factor R.  Results are therefore the
+// same canonical integers the reference produces.
+#ifndef PFC_ARITH_H
+#define PFC_ARITH_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define PFC_HD __host__ __device__ __forceinline__
+#else
+#define PFC_HD static inline
+#endif
+
+namespace pfc {
+
+struct Modulus {
+    uint32_t d1, d0;  // d = p << s, high and low words (d1 >= 2^31)
+    uint32_t v;       // 3-by-2 reciprocal of d
+    uint32_t s;       // shift, clz64(p)
+};
+
+PFC_HD uint64_t mod_d(const Modulus& m) { return ((uint64_t)m.d1 << 32) | m.d0; }
+
+PFC_HD uint32_t clz64(uint64_t x) { return (uint32_t)__builtin_clzll(x); }
+
+// 128-bit product high word of two 64-bit values, from 32-bit pieces.
+PFC_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
+    uint64_t a0 = (uint32_t)a, a1 = a >> 32, b0 = (uint32_t)b, b1 = b >> 32;
+    uint64_t p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0, p11 = a1 * b1;
+    uint64_t mid = (p00 >> 32) + (uint32_t)p01 + (uint32_t)p10;
+    return p11 + (p01 >> 32) + (p10 >> 32) + (mid >> 32);
+}
+
+// Remainder of the 3-word number <u2,u1,u0> by d, given <u2,u1> < d.
+// <q1,q0> estimates the quotient; r = <u1 - q1*d1, u0> - d0*q1 - d is then the
+// remainder for quotient q1 + 1, taken mod 2^64, and at most two corrections
+// by d follow (Algorithm 5 of Moller-Granlund).  d0*q1 + d is formed in one
+// 32x32+64 multiply-add; its carry out of 64 bits cancels mod 2^64.
+PFC_HD uint64_t div3by2_rem(uint32_t u2, uint32_t u1, uint32_t u0, const Modulus& m) {
+    const uint64_t qq = (uint64_t)m.v * u2 + (((uint64_t)u2 << 32) | u1);
+    const uint32_t q1 = (uint32_t)(qq >> 32), q0 = (uint32_t)qq;
+    const uint32_t r1 = u1 - q1 * m.d1;
+    const uint64_t d = mod_d(m);
+    uint64_t r = ((((uint64_t)r1) << 32) | u0) - ((uint64_t)m.d0 * q1 + d);
+    if ((uint32_t)(r >> 32) >= q0) r += d;
+    if (r >= d) r -= d;
+    return r;
+}
+
+// (a * n) mod d for a < d and any 32-bit n.  Keeps the shifted form.
+PFC_HD uint64_t mulmod32(uint64_t a, uint32_t n, const Modulus& m) {
+    const uint64_t lo = (uint64_t)(uint32_t)a * n;
+    const uint64_t hi = (uint64_t)(uint32_t)(a >> 32) * n + (lo >> 32);
+    return div3by2_rem((uint32_t)(hi >> 32), (uint32_t)hi, (uint32_t)lo, m);
+}
+
+// (a * 2^32) mod d for a < d.
+PFC_HD uint64_t shl32mod(uint64_t a, const Modulus& m) {
+    return div3by2_rem((uint32_t)(a >> 32), (uint32_t)a, 0u, m);
+}
+
+// (a + b) mod d for a, b < d (d >= 2^63, so a + b may carry out of 64 bits).
+PFC_HD uint64_t addmod(uint64_t a, uint64_t b, const Modulus& m) {
+    const uint64_t d = mod_d(m);
+    const uint64_t r = a + b;
+    return (r < a || r >= d) ? r - d : r;
+}
+
+// (a * b) mod d for a < d and any 64-bit b (b is a plain integer).
+PFC_HD uint64_t mulmod64(uint64_t a, uint64_t b, const Modulus& m) {
+    const uint64_t hi = shl32mod(mulmod32(a, (uint32_t)(b >> 32), m), m);
+    return addmod(hi, mulmod32(a, (uint32_t)b, m), m);
+}
+
+// Square of the residue a' = x << s: (x*x mod p) << s.
+PFC_HD uint64_t sqrmod(uint64_t a, const Modulus& m) { return mulmod64(a, a >> m.s, m); }
+
+// Builds the modulus for odd p >= 3.  The reciprocal comes from a double
+// estimate corrected against exact 128-bit products d*V <= 2^96-1 < d*(V+1).
+PFC_HD Modulus make_modulus(uint64_t p) {
+    Modulus m;
+    m.s = clz64(p);
+    const uint64_t d = p << m.s;
+    m.d1 = (uint32_t)(d >> 32);
+    m.d0 = (uint32_t)d;
+    // 2^96 / d lies in (2^32, 2^33); the double estimate is within a few units.
+    uint64_t V = (uint64_t)(79228162514264337593543950336.0 / (double)d);
+    // d*V as a 128-bit number compared with 2^96 - 1 = <0xFFFFFFFF, 2^64-1>.
+    for (;;) {
+        const uint64_t hi = mulhi64(d, V);
+        if (hi > 0xFFFFFFFFull) { --V; continue; }
+        const uint64_t lo = d * (V + 1);
+        const uint64_t hi1 = mulhi64(d, V + 1);
+        (void)lo;
+        if (hi1 <= 0xFFFFFFFFull) { ++V; continue; }
+        break;
+    }
+    m.v = (uint32_t)(V - (1ull << 32));
+    return m;
+}
+
+// R mod p in shifted form, R = 2^64: 2^(64+s) mod d.  2^64 mod d = 2^64 - d
+// because 2^63 < d < 2^64; the remaining factor 2^s is applied in two steps
+// of at most 2^31 each (s <= 62 for p >= 3).
+PFC_HD uint64_t mont_one(const Modulus& m) {
+    const uint64_t x = 0 - mod_d(m);
+    const uint32_t s1 = m.s < 31 ? m.s : 31;
+    return mulmod32(mulmod32(x, 1u << s1, m), 1u << (m.s - s1), m);
+}
+
+// True when d*(v + 2^32) <= 2^96 - 1 < d*(v + 2^32 + 1), i.e. the reciprocal
+// is exact for d.  Used as the device-side state check.
+PFC_HD bool modulus_ok(uint64_t p, const Modulus& m) {
+    if (!(p & 1) || m.s != clz64(p) || mod_d(m) != (p << m.s)) return false;
+    const uint64_t d = mod_d(m);
+    const uint64_t V = (uint64_t)m.v + (1ull << 32);
+    return mulhi64(d, V) <= 0xFFFFFFFFull && mulhi64(d, V + 1) > 0xFFFFFFFFull;
+}
+
+// Base-2 strong probable-prime test of odd N >= 3 (the test of
+// kernels/getsegprimes.cl:79-108, computed in the shifted representation).
+PFC_HD bool strong_prp2(uint64_t N) {
+    const Modulus m = make_modulus(N);
+    const uint64_t d = mod_d(m);
+    const uint64_t one = 1ull << m.s;          // 1 in shifted form
+    const uint64_t nmo = d - one;              // N - 1 in shifted form
+    const uint32_t t = (uint32_t)__builtin_ctzll(N - 1);
+    const uint64_t e = N >> t;                 // odd part of N - 1
+    // Bit below the leading one of e.  For e == 1 (N = 2^t + 1) this is 0 and
+    // the test starts from 2^1; the reference's 6-bit-masked shift instead
+    // raises 2 to 2^64+1, which gives the same verdict for every such N below
+    // 2^64 that survives the 7..113 sieve (257, 65537, 2^32+1, 2^40+1,
+    // 2^48+1, 2^56+1; checked in the tests).
+    uint64_t bit = (1ull << (63 - clz64(e))) >> 1;
+    uint64_t a = 2ull << m.s;                  // 2 (N >= 3, so 2 < N)
+    while (bit) {
+        a = sqrmod(a, m);
+        if (e & bit) a = addmod(a, a, m);
+        bit >>= 1;
+    }
+    if (a == one || a == nmo) return true;
+    for (uint32_t i = 1; i < t; ++i) {
+        a = sqrmod(a, m);
+        if (a == nmo) return true;
+    }
+    return false;
+}
+
+}  // namespace pfc
+
+#endif  // PFC_ARITH_H

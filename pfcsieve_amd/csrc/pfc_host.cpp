@@ -1,0 +1,776 @@
+// pfc_host.cpp -- host driver of libpfcsieve.so: devices, jobs, start-residue
+// tables, the segment loop and the factor harvest (include/pfc.h).
+//
+// This is the device-independent half of the reference's cl_sieve.cpp
+// (setupSearch 596-665, tables 824-1326, main loop 1328-1761, getResults
+// 450-593) and verifyprime.cpp, rebuilt over HIP.  No call exits the process.
+#include <hip/hip_runtime.h>
+
+#include <errno.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/pfc.h"
+#include "pfc_internal.h"
+
+using pfc::set_error;
+using pfc::hip_fail;
+
+typedef unsigned __int128 u128;
+
+// ====================================================================== primes
+
+namespace {
+
+// Primes in [lo, hi] (inclusive, hi < 2^32) by a segmented odd-only sieve of
+// Eratosthenes.  Stands in for primesieve_generate_primes (cl_sieve.cpp:850,
+// 996, 1000, 1158, 1498): prime lists are unique, so any exact sieve agrees.
+std::vector<uint32_t> primes_in(uint64_t lo, uint64_t hi) {
+    std::vector<uint32_t> out;
+    if (hi < 2 || lo > hi) return out;
+    if (lo <= 2) out.push_back(2);
+    uint64_t root = (uint64_t)sqrt((double)hi);
+    while (root * root > hi) --root;
+    while ((root + 1) * (root + 1) <= hi) ++root;
+    std::vector<uint8_t> small(root + 1, 0);
+    std::vector<uint32_t> base;
+    for (uint64_t i = 3; i <= root; i += 2) {
+        if (small[i]) continue;
+        base.push_back((uint32_t)i);
+        for (uint64_t j = i * i; j <= root; j += 2 * i) small[j] = 1;
+    }
+    const uint64_t SEG = 1u << 21;  // odd numbers per segment
+    std::vector<uint8_t> seg(SEG);
+    uint64_t start = lo < 3 ? 3 : (lo | 1);
+    for (uint64_t s = start; s <= hi; s += 2 * SEG) {
+        const uint64_t e = std::min<uint64_t>(hi, s + 2 * SEG - 2);  // last odd
+        const uint64_t cnt = (e - s) / 2 + 1;
+        std::fill(seg.begin(), seg.begin() + cnt, 0);
+        for (uint32_t q : base) {
+            const uint64_t q2 = (uint64_t)q * q;
+            if (q2 > e) break;
+            uint64_t f = q2 >= s ? q2 : ((s + q - 1) / q) * q;
+            if (!(f & 1)) f += q;
+            for (uint64_t x = f; x <= e; x += 2 * q) seg[(x - s) / 2] = 1;
+        }
+        for (uint64_t k = 0; k < cnt; ++k)
+            if (!seg[k]) out.push_back((uint32_t)(s + 2 * k));
+    }
+    return out;
+}
+
+// Setup table (include/pfc.h pfc_setup_table): exponents of every prime r <= N
+// in X(N) = N! (kind 0), N# (kind 1) or N!/N# (kind 2); group b holds the
+// primes with bit b of the exponent set, multiplied into chunks below 2^32.
+struct HostTable {
+    std::vector<uint32_t> chunks;
+    std::vector<uint32_t> groups;  // nbits + 1 offsets
+    uint32_t nbits = 0;
+};
+
+HostTable build_table(uint32_t kind, uint32_t N) {
+    HostTable T;
+    std::vector<uint32_t> pr = primes_in(2, N);
+    std::vector<uint32_t> ex(pr.size());
+    uint32_t maxe = 0;
+    for (size_t i = 0; i < pr.size(); ++i) {
+        uint32_t e = 1;
+        if (kind != 1) {  // Legendre: sum floor(N / r^k)  (cl_sieve.cpp:824-839)
+            e = 0;
+            for (uint64_t pk = pr[i]; pk <= N; pk *= pr[i]) e += (uint32_t)(N / pk);
+            if (kind == 2) e -= 1;
+        }
+        ex[i] = e;
+        maxe = std::max(maxe, e);
+    }
+    T.nbits = maxe ? 32 - __builtin_clz(maxe) : 0;
+    T.groups.assign(T.nbits + 1, 0);
+    for (uint32_t b = 0; b < T.nbits; ++b) {
+        T.groups[b] = (uint32_t)T.chunks.size();
+        uint64_t acc = 1;
+        for (size_t i = 0; i < pr.size(); ++i) {
+            if (!((ex[i] >> b) & 1u)) continue;
+            if (acc * pr[i] > 0xFFFFFFFFull) {
+                T.chunks.push_back((uint32_t)acc);
+                acc = 1;
+            }
+            acc *= pr[i];
+        }
+        if (acc > 1) T.chunks.push_back((uint32_t)acc);
+    }
+    if (T.nbits) T.groups[T.nbits] = (uint32_t)T.chunks.size();
+    return T;
+}
+
+// 64-bit Montgomery helpers for the host-side isPrime (R = 2^64).
+inline uint64_t h_inv(uint64_t p) {
+    uint64_t x = p;  // p*p == 1 mod 8 gives 3 bits; each step doubles them
+    for (int i = 0; i < 5; ++i) x *= 2 - p * x;
+    return x;
+}
+inline uint64_t h_redc(u128 t, uint64_t p, uint64_t pinv) {
+    const uint64_t m = (uint64_t)t * pinv;
+    const uint64_t mp = (uint64_t)(((u128)m * p) >> 64);
+    const uint64_t hi = (uint64_t)(t >> 64);
+    return hi >= mp ? hi - mp : hi - mp + p;
+}
+
+}  // namespace
+
+// ====================================================================== device
+
+struct pfc_device {
+    int ordinal = 0;
+    hipDeviceProp_t prop{};
+    pfc_tuning tuning{};
+    std::mutex mu;
+};
+
+static pfc_tuning default_tuning() {
+    pfc_tuning t;
+    t.primes_per_lane = 2;
+    t.block_size = 256;
+    t.walk_chunk = 1u << 16;
+    t.seg_candidates = 1u << 22;
+    return t;
+}
+
+extern "C" int pfc_device_open(int ordinal, pfc_device** out) {
+    pfc::g_err[0] = 0;
+    if (!out) return set_error(PFC_ERR_INVALID, "device_open: null out");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    if (ordinal < 0 || ordinal >= n) return set_error(PFC_ERR_INVALID, "device ordinal out of range");
+    pfc_device* d = new pfc_device();
+    d->ordinal = ordinal;
+    e = hipSetDevice(ordinal);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&d->prop, ordinal);
+    if (e != hipSuccess) {
+        delete d;
+        return hip_fail(e, "hipGetDeviceProperties");
+    }
+    d->tuning = default_tuning();
+    *out = d;
+    return PFC_OK;
+}
+
+extern "C" int pfc_device_close(pfc_device* dev) {
+    delete dev;
+    return PFC_OK;
+}
+
+extern "C" int pfc_device_info(const pfc_device* dev, char* name, size_t cap, int* cus,
+                               uint64_t* hbm) {
+    if (!dev) return set_error(PFC_ERR_INVALID, "device_info: null device");
+    if (name && cap) snprintf(name, cap, "%s", dev->prop.gcnArchName);
+    if (cus) *cus = dev->prop.multiProcessorCount;
+    if (hbm) *hbm = dev->prop.totalGlobalMem;
+    return PFC_OK;
+}
+
+extern "C" int pfc_set_tuning(pfc_device* dev, const pfc_tuning* t) {
+    if (!dev || !t) return set_error(PFC_ERR_INVALID, "set_tuning: null pointer");
+    pfc_tuning d = default_tuning();
+    pfc_tuning n = *t;
+    if (!n.primes_per_lane) n.primes_per_lane = d.primes_per_lane;
+    if (!n.block_size) n.block_size = d.block_size;
+    if (!n.walk_chunk) n.walk_chunk = d.walk_chunk;
+    if (!n.seg_candidates) n.seg_candidates = d.seg_candidates;
+    if (n.primes_per_lane != 1 && n.primes_per_lane != 2 && n.primes_per_lane != 4)
+        return set_error(PFC_ERR_INVALID, "primes_per_lane must be 1, 2 or 4");
+    if (n.block_size != 64 && n.block_size != 128 && n.block_size != 256)
+        return set_error(PFC_ERR_INVALID, "block_size must be 64, 128 or 256");
+    if (n.seg_candidates > (1u << 28)) return set_error(PFC_ERR_CAPACITY, "seg_candidates > 2^28");
+    std::lock_guard<std::mutex> g(dev->mu);
+    dev->tuning = n;
+    return PFC_OK;
+}
+
+extern "C" int pfc_get_tuning(const pfc_device* dev, pfc_tuning* t) {
+    if (!dev || !t) return set_error(PFC_ERR_INVALID, "get_tuning: null pointer");
+    *t = dev->tuning;
+    return PFC_OK;
+}
+
+extern "C" int pfc_probe_mul_rates(pfc_device* dev, double rates[3]) {
+    if (!dev || !rates) return set_error(PFC_ERR_INVALID, "probe: null pointer");
+    hipSetDevice(dev->ordinal);
+    return pfc::probe_mul_rates(dev->prop.multiProcessorCount, rates);
+}
+
+// ====================================================================== jobs
+
+namespace {
+
+// Validation of setupSearch (cl_sieve.cpp:600-641) plus the CLI ranges
+// (main.cpp:58-72).
+int validate(const pfc_job_desc* d) {
+    if (!d) return set_error(PFC_ERR_INVALID, "null job description");
+    const uint32_t m = d->modes;
+    if (!(m & 7u) || (m & ~7u)) return set_error(PFC_ERR_INVALID, "-! or -# or -c argument is required");
+    if ((m & PFC_MODE_PRIMORIAL) && (m & ~PFC_MODE_PRIMORIAL))
+        return set_error(PFC_ERR_INVALID, "Select only one test type!");
+    if (d->pmin == 0 || d->pmax == 0) return set_error(PFC_ERR_INVALID, "-p and -P arguments are required");
+    if (d->nmin == 0 || d->nmax == 0) return set_error(PFC_ERR_INVALID, "-n and -N arguments are required");
+    if (d->nmin > d->nmax) return set_error(PFC_ERR_INVALID, "nmin <= nmax is required");
+    if (d->pmin > d->pmax) return set_error(PFC_ERR_INVALID, "pmin <= pmax is required");
+    if (d->pmin < 3 || d->nmin < 101 || d->nmax > 0x7FFFFFFFu)
+        return set_error(PFC_ERR_INVALID, "out of range: 3 <= p, 101 <= n, N < 2^31");
+    return PFC_OK;
+}
+
+uint64_t default_fcap(const pfc_job_desc* d) {
+    if (d->factor_capacity) return d->factor_capacity;
+    return d->pmin < 0xFFFFFFFFull ? 30000000ull : 1000000ull;  // cl_sieve.cpp:645-647, main.cpp:215
+}
+
+// Montgomery-Vaughan: pi(x + y) - pi(x) <= 2y / ln y, plus margin for the
+// base-2 pseudoprimes and the primes below 114.
+uint64_t candidate_bound(uint64_t span) {
+    if (span < 64) return 96;
+    return (uint64_t)(2.1 * (double)span / log((double)span)) + 1024;
+}
+
+struct SegPlan {
+    uint64_t span;  // integers per segment
+    uint32_t cap;   // candidate capacity
+};
+
+SegPlan plan_segments(const pfc_job_desc* d, const pfc_tuning& t) {
+    const double lnp = log((double)std::max<uint64_t>(d->pmax, 16));
+    double span = (double)t.seg_candidates * lnp;
+    const double total = (double)(d->pmax - d->pmin);
+    if (span > total) span = total;
+    if (span < 1) span = 1;
+    SegPlan s;
+    s.span = (uint64_t)span;
+    uint64_t c = candidate_bound(s.span);
+    if (c > 0xF0000000ull) c = 0xF0000000ull;
+    s.cap = (uint32_t)c;
+    return s;
+}
+
+}  // namespace
+
+struct pfc_job {
+    pfc_device* dev = nullptr;
+    pfc_job_desc desc{};
+    pfc_tuning tuning{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    SegPlan plan{};
+    uint64_t fcap = 0;
+    uint64_t pos = 0;
+    // host tables
+    HostTable ta, tb;
+    std::vector<uint32_t> list;       // primorial multipliers
+    std::vector<uint32_t> composite;  // bitmap over n - nmin (C and F|C)
+    uint32_t jbegin = 0, jend = 0;    // walk step range
+    // device memory
+    uint32_t *d_ca = nullptr, *d_ga = nullptr, *d_cb = nullptr, *d_gb = nullptr;
+    uint32_t* d_list = nullptr;
+    uint32_t* d_comp = nullptr;
+    uint64_t* d_p = nullptr;
+    uint32_t* d_v = nullptr;
+    uint64_t *d_one = nullptr, *d_ra = nullptr, *d_rb = nullptr;
+    pfc_ctrl* d_ctrl = nullptr;
+    pfc_factor* d_fac = nullptr;
+    pfc_ctrl* h_ctrl = nullptr;  // pinned
+    // results
+    uint64_t dev_checksum = 0, primecount = 0, segments = 0;
+    double device_ms = 0;
+    std::vector<pfc_factor> factors;
+};
+
+namespace {
+
+void free_job_device(pfc_job* j) {
+    void* ptrs[] = {j->d_ca, j->d_ga, j->d_cb, j->d_gb, j->d_list, j->d_comp, j->d_p, j->d_v,
+                    j->d_one, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (j->h_ctrl) hipHostFree(j->h_ctrl);
+    if (j->ev0) hipEventDestroy(j->ev0);
+    if (j->ev1) hipEventDestroy(j->ev1);
+    if (j->stream) hipStreamDestroy(j->stream);
+}
+
+template <class T>
+int upload(T** dst, const std::vector<T>& src, hipStream_t st, const char* what) {
+    const size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)dst, bytes);
+    if (e != hipSuccess) return hip_fail(e, what);
+    if (!src.empty()) {
+        e = hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return hip_fail(e, what);
+    }
+    return PFC_OK;
+}
+
+// Step range and multiplier data of the walk (cl_sieve.cpp:1478-1499,
+// 996-1000, 1160-1170): F and C walk n in [nmin, nmax); P walks the primes in
+// [nmin, nmax-1] and requires nmax composite (cl_sieve.cpp:1058-1066).
+int build_walk_data(pfc_job* j) {
+    const pfc_job_desc& d = j->desc;
+    const uint32_t N = d.nmin - 1;
+    if (d.modes == PFC_MODE_PRIMORIAL) {
+        j->ta = build_table(1, N);
+        j->list = primes_in(d.nmin, d.nmax - 1);
+        std::vector<uint32_t> last = primes_in(d.nmax, d.nmax);
+        if (!last.empty()) return set_error(PFC_ERR_SIEVE, "CPU sieve failure.");
+        j->jbegin = 0;
+        j->jend = (uint32_t)j->list.size();
+        return PFC_OK;
+    }
+    j->jbegin = d.nmin;
+    j->jend = d.nmax;
+    if (d.modes & PFC_MODE_FACTORIAL) j->ta = build_table(0, N);
+    if (d.modes & PFC_MODE_COMPOSITORIAL) {
+        if (d.modes & PFC_MODE_FACTORIAL) j->tb = build_table(2, N);
+        else j->ta = build_table(2, N);
+        const uint32_t w = d.nmax - d.nmin;
+        j->composite.assign(w / 32 + 2, 0xFFFFFFFFu);
+        for (uint32_t q : primes_in(d.nmin, d.nmax - 1)) {
+            const uint32_t off = q - d.nmin;
+            j->composite[off >> 5] &= ~(1u << (off & 31));
+        }
+    }
+    return PFC_OK;
+}
+
+uint64_t footprint_of(const pfc_job_desc* d, const pfc_tuning& t) {
+    const SegPlan s = plan_segments(d, t);
+    const bool two = (d->modes & PFC_MODE_FACTORIAL) && (d->modes & PFC_MODE_COMPOSITORIAL);
+    uint64_t per = 8 + 4 + 8 + 8 + (two ? 8 : 0);
+    uint64_t bytes = (uint64_t)s.cap * per + default_fcap(d) * sizeof(pfc_factor) + sizeof(pfc_ctrl);
+    const uint64_t w = (uint64_t)(d->nmax - d->nmin);
+    if (d->modes & PFC_MODE_COMPOSITORIAL) bytes += w / 8 + 8;
+    if (d->modes & PFC_MODE_PRIMORIAL) bytes += 4 * (uint64_t)(1.3 * (double)w / log((double)d->nmax) + 64);
+    // start tables: about one 32-bit chunk per prime below nmin
+    bytes += 8 * (uint64_t)(1.3 * (double)d->nmin / log((double)d->nmin) + 64);
+    return bytes;
+}
+
+}  // namespace
+
+extern "C" int pfc_job_footprint(const pfc_device* dev, const pfc_job_desc* desc, uint64_t* bytes) {
+    pfc::g_err[0] = 0;
+    int rc = validate(desc);
+    if (rc) return rc;
+    if (!bytes) return set_error(PFC_ERR_INVALID, "footprint: null out");
+    *bytes = footprint_of(desc, dev ? dev->tuning : default_tuning());
+    return PFC_OK;
+}
+
+extern "C" int pfc_job_create(pfc_device* dev, const pfc_job_desc* desc, pfc_job** out) {
+    pfc::g_err[0] = 0;
+    if (!dev || !out) return set_error(PFC_ERR_INVALID, "job_create: null pointer");
+    int rc = validate(desc);
+    if (rc) return rc;
+    hipSetDevice(dev->ordinal);
+    pfc_job* j = new pfc_job();
+    j->dev = dev;
+    j->desc = *desc;
+    {
+        std::lock_guard<std::mutex> g(dev->mu);
+        j->tuning = dev->tuning;
+    }
+    j->plan = plan_segments(desc, j->tuning);
+    j->fcap = default_fcap(desc);
+    if (j->fcap > 0xFFFFFFFFull) j->fcap = 0xFFFFFFFFull;
+    j->pos = desc->pmin;
+    rc = build_walk_data(j);
+    if (rc) {
+        delete j;
+        return rc;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&j->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&j->ev1);
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "stream/event create");
+        free_job_device(j);
+        delete j;
+        return rc;
+    }
+    const uint32_t cap = j->plan.cap;
+    const bool two = (desc->modes & PFC_MODE_FACTORIAL) && (desc->modes & PFC_MODE_COMPOSITORIAL);
+    struct A { void** p; size_t bytes; };
+    A allocs[] = {{(void**)&j->d_p, (size_t)cap * 8},       {(void**)&j->d_v, (size_t)cap * 4},
+                  {(void**)&j->d_one, (size_t)cap * 8},     {(void**)&j->d_ra, (size_t)cap * 8},
+                  {(void**)&j->d_ctrl, sizeof(pfc_ctrl)},   {(void**)&j->d_fac, (size_t)j->fcap * sizeof(pfc_factor)}};
+    for (const A& a : allocs) {
+        e = hipMalloc(a.p, a.bytes);
+        if (e != hipSuccess) break;
+    }
+    if (e == hipSuccess && two) e = hipMalloc((void**)&j->d_rb, (size_t)cap * 8);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&j->h_ctrl, sizeof(pfc_ctrl), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        rc = set_error(PFC_ERR_NOMEM, "device allocation failed");
+        free_job_device(j);
+        delete j;
+        return rc;
+    }
+    rc = upload(&j->d_ca, j->ta.chunks, j->stream, "table A");
+    if (!rc) rc = upload(&j->d_ga, j->ta.groups, j->stream, "table A groups");
+    if (!rc && two) rc = upload(&j->d_cb, j->tb.chunks, j->stream, "table B");
+    if (!rc && two) rc = upload(&j->d_gb, j->tb.groups, j->stream, "table B groups");
+    if (!rc && !j->list.empty()) rc = upload(&j->d_list, j->list, j->stream, "multiplier list");
+    if (!rc && !j->composite.empty()) rc = upload(&j->d_comp, j->composite, j->stream, "composite map");
+    if (!rc) {
+        e = hipStreamSynchronize(j->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "table upload");
+    }
+    if (rc) {
+        free_job_device(j);
+        delete j;
+        return rc;
+    }
+    *out = j;
+    return PFC_OK;
+}
+
+// One p-segment [lo, hi): generate, setup, walk in chunks, check, then drain
+// the control block and factor records (cl_sieve.cpp:1563-1726, 450-494).
+static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
+    hipStream_t st = j->stream;
+    const uint32_t cap = j->plan.cap;
+    const pfc_job_desc& d = j->desc;
+    *overflow = false;
+    hipError_t e = hipMemsetAsync(j->d_ctrl, 0, sizeof(pfc_ctrl), st);
+    if (e != hipSuccess) return hip_fail(e, "ctrl clear");
+    int rc = pfc::launch_generate(lo, hi, j->d_p, cap, j->d_ctrl, st);
+    if (rc) return rc;
+    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits};
+    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits};
+    rc = pfc::launch_setup(j->d_p, j->d_ctrl, j->d_v, j->d_one, j->d_ra, j->d_rb, &ta,
+                           j->d_rb ? &tb : nullptr, cap, st);
+    if (rc) return rc;
+    const uint32_t chunk = std::max<uint32_t>(j->tuning.walk_chunk, 1);
+    for (uint64_t a = j->jbegin; a < j->jend; a += chunk) {
+        const uint32_t b = (uint32_t)std::min<uint64_t>(j->jend, a + chunk);
+        rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, d.nmin, j->d_p, j->d_v,
+                              j->d_one, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
+                              cap, j->tuning.primes_per_lane, j->tuning.block_size, st);
+        if (rc) return rc;
+    }
+    rc = pfc::launch_check(d.modes, d.nmax, j->d_p, j->d_v, j->d_one, j->d_ra, j->d_rb, j->d_ctrl,
+                           cap, st);
+    if (rc) return rc;
+    e = hipMemcpyAsync(j->h_ctrl, j->d_ctrl, sizeof(pfc_ctrl), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "segment");
+    const pfc_ctrl c = *j->h_ctrl;
+    if (c.flags & 1u) {
+        *overflow = true;
+        return PFC_OK;
+    }
+    if (c.flags & 2u) return set_error(PFC_ERR_VALIDATION, "error: gpu validation failure");
+    if (c.nfactor > j->fcap) {
+        char msg[128];
+        snprintf(msg, sizeof(msg), "Error: number of results (%u) overflowed array.", c.nfactor);
+        return set_error(PFC_ERR_FACTOR_OVERFLOW, msg);
+    }
+    if (c.nfactor) {
+        const size_t old = j->factors.size();
+        j->factors.resize(old + c.nfactor);
+        e = hipMemcpy(j->factors.data() + old, j->d_fac, (size_t)c.nfactor * sizeof(pfc_factor),
+                      hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "factor copy");
+    }
+    j->dev_checksum += c.checksum;
+    j->primecount += c.primecount;
+    j->segments += 1;
+    return PFC_OK;
+}
+
+extern "C" int pfc_job_run(pfc_job* j, uint64_t max_segments) {
+    pfc::g_err[0] = 0;
+    if (!j) return set_error(PFC_ERR_INVALID, "job_run: null job");
+    hipSetDevice(j->dev->ordinal);
+    hipEventRecord(j->ev0, j->stream);
+    uint64_t done = 0;
+    uint64_t span = j->plan.span;
+    while (j->pos < j->desc.pmax && (max_segments == 0 || done < max_segments)) {
+        uint64_t hi = j->pos + span;
+        if (hi > j->desc.pmax || hi < j->pos) hi = j->desc.pmax;
+        bool overflow = false;
+        int rc = run_segment(j, j->pos, hi, &overflow);
+        if (rc) return rc;
+        if (overflow) {  // more candidates than the bound: split the segment
+            if (span <= 1) return set_error(PFC_ERR_CAPACITY, "candidate buffer overflow");
+            span /= 2;
+            continue;
+        }
+        j->pos = hi;
+        ++done;
+    }
+    hipEventRecord(j->ev1, j->stream);
+    hipEventSynchronize(j->ev1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, j->ev0, j->ev1);
+    j->device_ms += ms;
+    return PFC_OK;
+}
+
+extern "C" int pfc_job_position(const pfc_job* j, uint64_t* p) {
+    if (!j || !p) return set_error(PFC_ERR_INVALID, "job_position: null pointer");
+    *p = j->pos;
+    return PFC_OK;
+}
+
+extern "C" int pfc_job_seek(pfc_job* j, uint64_t p, uint64_t dev_checksum, uint64_t primecount) {
+    if (!j) return set_error(PFC_ERR_INVALID, "job_seek: null job");
+    if (p < j->desc.pmin || p > j->desc.pmax) return set_error(PFC_ERR_INVALID, "seek outside [pmin, pmax]");
+    j->pos = p;
+    j->dev_checksum = dev_checksum;
+    j->primecount = primecount;
+    return PFC_OK;
+}
+
+extern "C" int pfc_job_stats_get(const pfc_job* j, pfc_job_stats* s) {
+    if (!j || !s) return set_error(PFC_ERR_INVALID, "job_stats: null pointer");
+    s->dev_checksum = j->dev_checksum;
+    s->primecount = j->primecount;
+    s->raw_factors = j->factors.size();
+    s->segments = j->segments;
+    s->device_ms = j->device_ms;
+    return PFC_OK;
+}
+
+static bool factor_less(const pfc_factor& x, const pfc_factor& y) {
+    if (x.p != y.p) return x.p < y.p;
+    const int32_t nx = x.nc < 0 ? -x.nc : x.nc, ny = y.nc < 0 ? -y.nc : y.nc;
+    if (nx != ny) return nx < ny;
+    return x.type < y.type;
+}
+
+extern "C" int pfc_job_factors(const pfc_job* j, pfc_factor* out, size_t cap, size_t* n) {
+    if (!j || !n) return set_error(PFC_ERR_INVALID, "job_factors: null pointer");
+    *n = j->factors.size();
+    if (!out) return PFC_OK;
+    std::vector<pfc_factor> f = j->factors;
+    std::sort(f.begin(), f.end(), factor_less);
+    const size_t k = std::min(cap, f.size());
+    if (k) memcpy(out, f.data(), k * sizeof(pfc_factor));
+    return PFC_OK;
+}
+
+extern "C" int pfc_job_clear_factors(pfc_job* j) {
+    if (!j) return set_error(PFC_ERR_INVALID, "job_clear_factors: null job");
+    j->factors.clear();
+    return PFC_OK;
+}
+
+extern "C" int pfc_job_destroy(pfc_job* j) {
+    if (!j) return PFC_OK;
+    hipSetDevice(j->dev->ordinal);
+    if (j->stream) hipStreamSynchronize(j->stream);
+    free_job_device(j);
+    delete j;
+    return PFC_OK;
+}
+
+extern "C" int pfc_batch_run(pfc_job** jobs, size_t njobs) {
+    for (size_t i = 0; i < njobs; ++i) {
+        int rc = pfc_job_run(jobs[i], 0);
+        if (rc) return rc;
+    }
+    return PFC_OK;
+}
+
+// ====================================================================== harvest
+
+extern "C" int pfc_is_prime(uint64_t p) {
+    // Bases of verifyprime.cpp:111 (deterministic below 2^64).
+    static const uint32_t bases[7] = {2, 325, 9375, 28178, 450775, 9780504, 1795265022};
+    if (p < 2 || !(p & 1)) return p == 2;
+    if (p < 9) return p != 1;
+    const uint64_t pinv = h_inv(p);
+    const uint64_t one = (uint64_t)(((u128)1 << 64) % p);
+    const uint64_t r2 = (uint64_t)(((u128)one * one) % p);
+    const uint64_t pm1 = p - one;  // Montgomery form of -1
+    const int t = __builtin_ctzll(p - 1);
+    const uint64_t dd = (p - 1) >> t;
+    for (int i = 0; i < 7; ++i) {
+        uint64_t b = bases[i] % p;
+        if (b == 0) continue;
+        uint64_t base = h_redc((u128)b * r2, p, pinv);
+        uint64_t a = one;
+        for (int k = 63 - __builtin_clzll(dd); k >= 0; --k) {
+            a = h_redc((u128)a * a, p, pinv);
+            if ((dd >> k) & 1) a = h_redc((u128)a * base, p, pinv);
+        }
+        if (a == one || a == pm1) continue;
+        bool ok = false;
+        for (int s = 1; s < t && !ok; ++s) {
+            a = h_redc((u128)a * a, p, pinv);
+            ok = a == pm1;
+        }
+        if (!ok) return 0;
+    }
+    return 1;
+}
+
+// X(n) mod p by direct products of the integers, primes or composites up to
+// n, stopping once the product is 0 (then it can no longer be +-1).
+static uint64_t slow_residue(uint64_t p, uint32_t n, int32_t type, const std::vector<uint8_t>& isp) {
+    uint64_t r = 1 % p;
+    for (uint32_t i = 2; i <= n && r; ++i) {
+        if (type == PFC_PRIMORIAL && !isp[i]) continue;
+        if (type == PFC_COMPOSITORIAL && isp[i]) continue;
+        r = p < 0xFFFFFFFFull ? (r * i) % p : (uint64_t)(((u128)r * i) % p);
+    }
+    return r;
+}
+
+extern "C" int pfc_verify_factor(uint64_t p, uint32_t n, int32_t c, int32_t type) {
+    std::vector<uint8_t> isp(n + 1, 0);
+    if (type != PFC_FACTORIAL)
+        for (uint32_t q : primes_in(2, n)) isp[q] = 1;
+    const uint64_t r = slow_residue(p, n, type, isp);
+    return (r == 1 && c == -1) || (r == p - 1 && c == 1);
+}
+
+extern "C" int pfc_harvest(pfc_factor* f, size_t n, int verify_threads, uint64_t* checksum,
+                           size_t* nkept) {
+    pfc::g_err[0] = 0;
+    if ((!f && n) || !checksum || !nkept) return set_error(PFC_ERR_INVALID, "harvest: null pointer");
+    std::sort(f, f + n, factor_less);
+    if (verify_threads > 0 && n) {
+        uint32_t maxn = 0;
+        bool need_primes = false;
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t a = (uint32_t)(f[i].nc < 0 ? -f[i].nc : f[i].nc);
+            maxn = std::max(maxn, a);
+            need_primes |= f[i].type != PFC_FACTORIAL;
+        }
+        std::vector<uint8_t> isp(need_primes ? maxn + 1 : 1, 0);
+        if (need_primes)
+            for (uint32_t q : primes_in(2, maxn)) isp[q] = 1;
+        std::atomic<size_t> next(0);
+        std::atomic<long long> bad(-1);
+        auto worker = [&]() {
+            for (;;) {
+                const size_t i = next.fetch_add(64);
+                if (i >= n || bad.load() >= 0) return;
+                for (size_t k = i; k < std::min(n, i + 64); ++k) {
+                    const uint32_t a = (uint32_t)(f[k].nc < 0 ? -f[k].nc : f[k].nc);
+                    const int32_t c = f[k].nc < 0 ? -1 : 1;
+                    const uint64_t r = slow_residue(f[k].p, a, f[k].type, isp);
+                    if (!((r == 1 && c == -1) || (r == f[k].p - 1 && c == 1))) {
+                        long long exp = -1;
+                        bad.compare_exchange_strong(exp, (long long)k);
+                        return;
+                    }
+                }
+            }
+        };
+        const int nt = std::max(1, std::min(verify_threads, 128));
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(worker);
+        for (auto& x : th) x.join();
+        if (bad.load() >= 0) {
+            const pfc_factor& b = f[bad.load()];
+            const uint32_t a = (uint32_t)(b.nc < 0 ? -b.nc : b.nc);
+            const char* form = b.type == PFC_FACTORIAL ? "!" : b.type == PFC_PRIMORIAL ? "#" : "!/#";
+            char msg[200];
+            snprintf(msg, sizeof(msg), "CPU factor verification failed!  %llu is not a factor of %u%s%+d",
+                     (unsigned long long)b.p, a, form, b.nc < 0 ? -1 : 1);
+            return set_error(PFC_ERR_VERIFY, msg);
+        }
+    }
+    uint64_t last = 0;
+    size_t k = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (f[i].p == last || pfc_is_prime(f[i].p)) {  // cl_sieve.cpp:561
+            last = f[i].p;
+            const uint32_t a = (uint32_t)(f[i].nc < 0 ? -f[i].nc : f[i].nc);
+            const uint32_t c = f[i].nc < 0 ? 0xFFFFFFFFu : 1u;
+            *checksum += (uint64_t)(uint32_t)(a + c);  // cl_sieve.cpp:583
+            f[k++] = f[i];
+        }
+    }
+    *nkept = k;
+    return PFC_OK;
+}
+
+extern "C" int pfc_format_factor(const pfc_factor* f, char* buf, size_t cap) {
+    if (!f || !buf) return set_error(PFC_ERR_INVALID, "format: null pointer");
+    const uint32_t a = (uint32_t)(f->nc < 0 ? -f->nc : f->nc);
+    const int c = f->nc < 0 ? -1 : 1;
+    const char* form = f->type == PFC_FACTORIAL ? "!" : f->type == PFC_PRIMORIAL ? "#" : "!/#";
+    return snprintf(buf, cap, "%llu | %u%s%+d\n", (unsigned long long)f->p, a, form, c);
+}
+
+// putil.c:34-116 grammar: strtoull base 0, then one suffix: K M G T P (x1000
+// each), eN (x10^N), k m g t p (x2^10 each), bN (x2^N).  0 ok, -1 parse
+// error, -2 out of [lo, hi].
+extern "C" int pfc_parse_uint64(const char* s, uint64_t lo, uint64_t hi, uint64_t* out) {
+    if (!s || !out) return -1;
+    char* tail = nullptr;
+    errno = 0;
+    uint64_t v = strtoull(s, &tail, 0);
+    if (errno != 0 || v > hi) return -2;
+    const char c = *tail;
+    static const char dec[] = "KMGTP", bin[] = "kmgtp";
+    const char* pd = c ? strchr(dec, c) : nullptr;
+    const char* pb = c ? strchr(bin, c) : nullptr;
+    if (pd) {
+        if (tail[1] != '\0') return -1;
+        for (int k = 0; k <= (int)(pd - dec); ++k) {
+            if (v > hi / 1000) return -2;
+            v *= 1000;
+        }
+    } else if (pb) {
+        if (tail[1] != '\0') return -1;
+        const int sh = 10 * (int)(pb - bin + 1);
+        if (v > (hi >> sh)) return -2;
+        v <<= sh;
+    } else if (c == 'e' || c == 'E' || c == 'b' || c == 'B') {
+        errno = 0;
+        char* t2 = nullptr;
+        unsigned long ex = strtoul(tail + 1, &t2, 0);
+        if (errno != 0) return -2;
+        if (*t2 != '\0') return -1;
+        const bool ten = (c == 'e' || c == 'E');
+        while (ex-- > 0) {
+            if (ten) {
+                if (v > hi / 10) return -2;
+                v *= 10;
+            } else {
+                if (v > (hi >> 1)) return -2;
+                v <<= 1;
+            }
+        }
+    } else if (c != '\0') {
+        return -1;
+    }
+    if (v < lo) return -2;
+    *out = v;
+    return 0;
+}
+
+// ====================================================================== tables (C ABI)
+
+extern "C" int pfc_build_setup_table(uint32_t kind, uint32_t N, uint32_t* chunks, size_t chunk_cap,
+                                     size_t* nchunks, uint32_t* groups, size_t group_cap,
+                                     uint32_t* nbits) {
+    if (kind > 2 || !nchunks || !nbits) return set_error(PFC_ERR_INVALID, "build_setup_table: bad args");
+    HostTable T = build_table(kind, N);
+    *nchunks = T.chunks.size();
+    *nbits = T.nbits;
+    if (chunks) memcpy(chunks, T.chunks.data(), std::min(chunk_cap, T.chunks.size()) * 4);
+    if (groups) memcpy(groups, T.groups.data(), std::min(group_cap, T.groups.size()) * 4);
+    return PFC_OK;
+}

@@ -1,0 +1,566 @@
+// pfc_kernels.hip -- gfx950 kernels of the sieve path and their stage-level
+// C entry points (declared in include/pfc.h).
+//
+//   k_generate : candidate primes of a p-segment (wheel-30 + sieve 7..113 +
+//                base-2 strong PRP), replacing kernels/getsegprimes.cl:464-526
+//                and kernels/addsmallprimes.cl:21-40.
+//   k_setup    : per-candidate reciprocal, R mod p and start residues
+//                Mont(X(nmin-1)), replacing kernels/setup.cl:49-288.
+//   k_walk     : the residue walk, replacing kernels/iterate.cl:59-198.
+//   k_check    : the checksum fold, replacing kernels/check.cl:45-204.
+//
+// Device state of a segment is structure-of-arrays, one entry per candidate:
+// p (u64), v (u32 reciprocal), one (u64, R mod p shifted), res_a, res_b (u64
+// shifted residues).  Candidates are appended in no particular order; every
+// later result (sums, sorted factor lists) is order independent.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "../../include/pfc.h"
+#include "pfc_arith.h"
+#include "pfc_internal.h"
+
+using pfc::Modulus;
+
+namespace {
+
+// ------------------------------------------------------------------ generate
+
+// Bit j of kPattern[i] is set iff sieve prime i divides j (j < 64): shifted
+// left by the first multiple's offset it marks every multiple in a 64-wide
+// window of consecutive integers.
+struct SieveTables {
+    uint32_t prime[27];
+    uint64_t pattern[27];
+};
+
+constexpr SieveTables make_sieve_tables() {
+    SieveTables t{};
+    const uint32_t ps[27] = {7,  11, 13, 17, 19, 23, 29, 31,  37,  41,  43,  47,  53, 59,
+                             61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113};
+    for (int i = 0; i < 27; ++i) {
+        t.prime[i] = ps[i];
+        uint64_t m = 0;
+        for (uint32_t j = 0; j < 64; j += ps[i]) m |= 1ull << j;
+        t.pattern[i] = m;
+    }
+    return t;
+}
+
+__constant__ SieveTables c_sieve = make_sieve_tables();
+
+// Mask of the integers coprime to 30 in a 60-wide window starting at a
+// multiple of 30: offsets 1,7,11,13,17,19,23,29 and the same +30.
+constexpr uint64_t kWheel60 = (1ull << 1) | (1ull << 7) | (1ull << 11) | (1ull << 13) |
+                              (1ull << 17) | (1ull << 19) | (1ull << 23) | (1ull << 29) |
+                              (1ull << 31) | (1ull << 37) | (1ull << 41) | (1ull << 43) |
+                              (1ull << 47) | (1ull << 49) | (1ull << 53) | (1ull << 59);
+
+constexpr int kGenBlock = 256;
+constexpr int kGenSpan = 60;                      // integers per thread
+constexpr int kGenLds = kGenBlock * 16;           // at most 16 survivors per thread
+
+// Appends values to out[] with one atomic per wavefront.
+__device__ __forceinline__ void wave_append(bool keep, uint64_t value, uint64_t* out, uint32_t cap,
+                                            pfc_ctrl* ctrl) {
+    const uint64_t ballot = __ballot(keep);
+    if (ballot == 0) return;
+    const uint32_t lane = __lane_id();
+    const uint32_t rank = __popcll(ballot & ((1ull << lane) - 1));
+    const int leader = __ffsll((unsigned long long)ballot) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(&ctrl->ncand, (uint32_t)__popcll(ballot));
+    base = __shfl(base, leader);
+    if (keep) {
+        const uint32_t idx = base + rank;
+        if (idx < cap) out[idx] = value;
+        else atomicOr(&ctrl->flags, 1u);
+    }
+}
+
+// Block b covers [base30 + b*15360, +15360), thread t the 60 integers from
+// base30 + b*15360 + 60*t; only integers in [lo, hi) are kept.  Survivors of
+// the wheel and the 7..113 sieve are compacted in LDS, then PRP-tested by the
+// whole block (the reference's shape, getsegprimes.cl:477-517, with the LDS
+// list bounded by construction: at most 16 survivors per thread).
+__global__ __launch_bounds__(kGenBlock) void k_generate(uint64_t base30, uint64_t lo, uint64_t hi,
+                                                       uint64_t* out, uint32_t cap,
+                                                       pfc_ctrl* ctrl) {
+    __shared__ uint64_t s_list[kGenLds];
+    __shared__ uint32_t s_mod[27];
+    __shared__ uint32_t s_count;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t block_base = base30 + (uint64_t)blockIdx.x * (kGenBlock * kGenSpan);
+    if (tid < 27) s_mod[tid] = (uint32_t)(block_base % c_sieve.prime[tid]);
+    if (tid == 0) s_count = 0;
+    __syncthreads();
+
+    const uint64_t start = block_base + (uint64_t)tid * kGenSpan;
+    uint64_t cand = kWheel60;
+    // Range clip: keep offsets j with lo <= start + j < hi.
+    if (start < lo) {
+        const uint64_t skip = lo - start;
+        cand = skip >= 64 ? 0 : cand & (~0ull << skip);
+    }
+    if (hi <= start) {
+        cand = 0;
+    } else if (hi - start < 64) {
+        cand &= (1ull << (hi - start)) - 1;
+    }
+    if (cand) {
+        uint64_t hit = 0;
+        const uint32_t toff = tid * kGenSpan;
+#pragma unroll
+        for (int i = 0; i < 27; ++i) {
+            const uint32_t q = c_sieve.prime[i];
+            const uint32_t r = (s_mod[i] + toff) % q;         // start mod q
+            const uint32_t j0 = r ? q - r : 0;                  // first multiple's offset
+            hit |= c_sieve.pattern[i] << j0;
+        }
+        cand &= ~hit;
+        while (cand) {
+            const uint32_t j = __ffsll((unsigned long long)cand) - 1;
+            cand &= cand - 1;
+            s_list[atomicAdd(&s_count, 1u)] = start + j;
+        }
+    }
+    __syncthreads();
+    const uint32_t n = s_count;
+    // Whole wavefronts iterate together so the append can use ballots.
+    for (uint32_t i0 = 0; i0 < n; i0 += kGenBlock) {
+        const uint32_t i = i0 + tid;
+        bool keep = false;
+        uint64_t c = 0;
+        if (i < n) {
+            c = s_list[i];
+            keep = pfc::strong_prp2(c);
+        }
+        wave_append(keep, c, out, cap, ctrl);
+    }
+}
+
+// Primes below 114 in [lo, hi) (addsmallprimes.cl:25-38).  They are divisible
+// by a sieve prime, so k_generate never emits them.
+__global__ void k_small_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint32_t cap,
+                               pfc_ctrl* ctrl) {
+    const uint32_t ps[29] = {3,  5,  7,  11, 13, 17, 19, 23, 29, 31,  37,  41,  43,  47, 53,
+                             59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113};
+    const uint32_t i = threadIdx.x;
+    bool keep = i < 29 && ps[i] >= lo && ps[i] < hi;
+    wave_append(keep, keep ? ps[i] : 0, out, cap, ctrl);
+}
+
+// ------------------------------------------------------------------ setup
+
+// Start residue of one table: Horner over exponent bits, high bit first,
+// acc = acc^2 * prod(chunks of group b), starting from 1; returned multiplied
+// by R mod p so that it is the Montgomery form (X*R mod p) << s.
+__device__ __forceinline__ uint64_t horner_start(const uint32_t* __restrict__ chunks,
+                                                 const uint32_t* __restrict__ groups,
+                                                 uint32_t nbits, const Modulus& m, uint64_t one) {
+    uint64_t acc = 1ull << m.s;
+    for (int b = (int)nbits - 1; b >= 0; --b) {
+        if (b != (int)nbits - 1) acc = pfc::sqrmod(acc, m);
+        const uint32_t g0 = groups[b], g1 = groups[b + 1];
+        for (uint32_t k = g0; k < g1; ++k) acc = pfc::mulmod32(acc, chunks[k], m);
+    }
+    return pfc::mulmod64(acc, one >> m.s, m);
+}
+
+__global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
+                                               const pfc_ctrl* __restrict__ ctrl,
+                                               uint32_t* __restrict__ V, uint64_t* __restrict__ ONE,
+                                               uint64_t* __restrict__ RA, uint64_t* __restrict__ RB,
+                                               const uint32_t* __restrict__ ca,
+                                               const uint32_t* __restrict__ ga, uint32_t nba,
+                                               const uint32_t* __restrict__ cb,
+                                               const uint32_t* __restrict__ gb, uint32_t nbb,
+                                               uint32_t cap) {
+    uint32_t n = ctrl->ncand;
+    if (n > cap) n = cap;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint64_t p = P[i];
+        const Modulus m = pfc::make_modulus(p);
+        const uint64_t one = pfc::mont_one(m);
+        V[i] = m.v;
+        ONE[i] = one;
+        RA[i] = horner_start(ca, ga, nba, m, one);
+        if (RB) RB[i] = horner_start(cb, gb, nbb, m, one);
+    }
+}
+
+// ------------------------------------------------------------------ walk
+
+__device__ __noinline__ void record_factor(pfc_factor* fac, uint32_t fcap, pfc_ctrl* ctrl,
+                                           uint64_t p, uint32_t n, bool plus, int32_t type) {
+    const uint32_t idx = atomicAdd(&ctrl->nfactor, 1u);
+    if (idx < fcap) {
+        pfc_factor f;
+        f.p = p;
+        f.nc = plus ? (int32_t)n : -(int32_t)n;
+        f.type = type;
+        fac[idx] = f;
+    }
+}
+
+struct WalkArgs {
+    uint32_t j0, j1;                  // multiplier steps [j0, j1)
+    const uint32_t* list;             // P, C: multipliers
+    const uint32_t* composite;        // F|C: composite bitmap, bit (n - nbase)
+    uint32_t nbase;
+    const uint64_t* P;
+    const uint32_t* V;
+    const uint64_t* ONE;
+    uint64_t* RA;
+    uint64_t* RB;
+    pfc_ctrl* ctrl;
+    pfc_factor* fac;
+    uint32_t fcap;
+    uint32_t cap;
+};
+
+// One lane owns K candidates (indices lane*K .. lane*K+K-1) and keeps their
+// residues in VGPRs for the whole launch.  The multiplier stream is uniform
+// across the wavefront.  MODE: 0 = F (n = j), 1 = P (n = list[j]),
+// 2 = F|C (n = j; composite n also multiplies res_b), 3 = C (n = j, composite
+// n only).  Composite flags come from a bitmap over n - nbase.
+template <int MODE, int K>
+__global__ __launch_bounds__(256) void k_walk(const WalkArgs A) {
+    uint32_t n_c = A.ctrl->ncand;
+    if (n_c > A.cap) n_c = A.cap;
+    const uint32_t lane_g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t first = lane_g * K;
+    if (first >= n_c) return;
+
+    Modulus m[K];
+    uint64_t ra[K], rb[K], one[K], nmo[K];
+    bool live[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t idx = first + k;
+        live[k] = idx < n_c;
+        const uint32_t src = live[k] ? idx : first;
+        const uint64_t p = A.P[src];
+        m[k].s = pfc::clz64(p);
+        const uint64_t d = p << m[k].s;
+        m[k].d1 = (uint32_t)(d >> 32);
+        m[k].d0 = (uint32_t)d;
+        m[k].v = A.V[src];
+        one[k] = A.ONE[src];
+        nmo[k] = d - one[k];
+        ra[k] = A.RA[src];
+        rb[k] = (MODE == 2) ? A.RB[src] : 0;
+    }
+    const int32_t type_a = MODE == 1 ? PFC_PRIMORIAL : MODE == 3 ? PFC_COMPOSITORIAL : PFC_FACTORIAL;
+
+    for (uint32_t j = A.j0; j < A.j1; ++j) {
+        // readfirstlane pins the multiplier to one 32-bit SGPR (the loop
+        // counter would otherwise be widened to 64 bits).
+        const uint32_t n = __builtin_amdgcn_readfirstlane((MODE == 1) ? A.list[j] : j);
+        if (MODE == 3) {
+            const uint32_t off = n - A.nbase;
+            if (!((A.composite[off >> 5] >> (off & 31)) & 1u)) continue;
+        }
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            ra[k] = pfc::mulmod32(ra[k], n, m[k]);
+            hit |= (ra[k] == one[k]) | (ra[k] == nmo[k]);
+        }
+        if (hit) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (live[k] && (ra[k] == one[k] || ra[k] == nmo[k]))
+                    record_factor(A.fac, A.fcap, A.ctrl, pfc::mod_d(m[k]) >> m[k].s, n,
+                                  ra[k] == nmo[k], type_a);
+        }
+        if (MODE == 2) {
+            const uint32_t off = n - A.nbase;
+            if ((A.composite[off >> 5] >> (off & 31)) & 1u) {
+                bool hitc = false;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    rb[k] = pfc::mulmod32(rb[k], n, m[k]);
+                    hitc |= (rb[k] == one[k]) | (rb[k] == nmo[k]);
+                }
+                if (hitc) {
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+                        if (live[k] && (rb[k] == one[k] || rb[k] == nmo[k]))
+                            record_factor(A.fac, A.fcap, A.ctrl, pfc::mod_d(m[k]) >> m[k].s, n,
+                                          rb[k] == nmo[k], PFC_COMPOSITORIAL);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (!live[k]) continue;
+        A.RA[first + k] = ra[k];
+        if (MODE == 2) A.RB[first + k] = rb[k];
+    }
+}
+
+// ------------------------------------------------------------------ check
+
+// Term per candidate (check.cl:59,115,161): F or C: Mont(X) + Mont(nmax-1);
+// F|C: Mont(F) + Mont(C) + Mont(nmax-1); P: Mont(X).  Block sums are added to
+// ctrl->checksum with one 64-bit atomic per block (wrapping, order free).
+__global__ __launch_bounds__(256) void k_check(uint32_t modes, uint32_t nmax,
+                                               const uint64_t* __restrict__ P,
+                                               const uint32_t* __restrict__ V,
+                                               const uint64_t* __restrict__ ONE,
+                                               const uint64_t* __restrict__ RA,
+                                               const uint64_t* __restrict__ RB, pfc_ctrl* ctrl,
+                                               uint32_t cap) {
+    __shared__ uint64_t s_part[256 / 64];
+    uint32_t n = ctrl->ncand;
+    const uint32_t ncap = n > cap ? cap : n;
+    uint64_t sum = 0;
+    bool bad = false;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ncap; i += gridDim.x * blockDim.x) {
+        const uint64_t p = P[i];
+        Modulus m;
+        m.s = pfc::clz64(p);
+        const uint64_t d = p << m.s;
+        m.d1 = (uint32_t)(d >> 32);
+        m.d0 = (uint32_t)d;
+        m.v = V[i];
+        bad |= !pfc::modulus_ok(p, m);
+        const uint64_t a = RA[i] >> m.s;
+        if (modes & PFC_MODE_PRIMORIAL) {
+            sum += a;
+        } else {
+            const uint64_t last = pfc::mulmod32(ONE[i], nmax - 1, m) >> m.s;
+            sum += a + last;
+            if ((modes & PFC_MODE_FACTORIAL) && (modes & PFC_MODE_COMPOSITORIAL)) sum += RB[i] >> m.s;
+        }
+    }
+    if (bad) atomicOr(&ctrl->flags, 2u);
+    // Wavefront sum, then one atomic per block.
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off);
+    if (__lane_id() == 0) s_part[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w) t += s_part[w];
+        atomicAdd((unsigned long long*)&ctrl->checksum, (unsigned long long)t);
+        if (blockIdx.x == 0) atomicAdd((unsigned long long*)&ctrl->primecount, (unsigned long long)n);
+    }
+}
+
+// ------------------------------------------------------------------ probe
+
+// Independent chains of one multiply class per lane (8 chains x 16 unrolled).
+template <int OP>
+__global__ __launch_bounds__(256) void k_probe(uint32_t* out, uint32_t c, int iters) {
+    uint32_t x[8];
+    uint64_t y[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        x[i] = threadIdx.x + i * 77u + c;
+        y[i] = x[i] * 3ull + 5;
+    }
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if constexpr (OP == 0) y[i] = (uint64_t)(uint32_t)y[i] * c + y[i];
+                if constexpr (OP == 1) asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(x[i]) : "v"(c));
+                if constexpr (OP == 2) x[i] = __umulhi(x[i], c);
+            }
+        }
+    }
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc += x[i] + (uint32_t)y[i] + (uint32_t)(y[i] >> 32);
+    out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+}  // namespace
+
+// ================================================================== C entry points
+
+namespace pfc {
+
+thread_local char g_err[512];
+
+int set_error(int code, const char* msg) {
+    snprintf(g_err, sizeof(g_err), "%s", msg);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+    return PFC_ERR_HIP;
+}
+
+int launch_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pfc_ctrl* d_ctrl,
+                    hipStream_t st) {
+    if (hi <= lo) return PFC_OK;
+    if (lo < 114) {
+        const uint64_t h = hi < 114 ? hi : 114;
+        hipLaunchKernelGGL(k_small_primes, dim3(1), dim3(64), 0, st, lo, h, d_p, cap, d_ctrl);
+        lo = h;
+        if (hi <= lo) return PFC_OK;
+    }
+    const uint64_t base30 = lo - lo % 30;
+    const uint64_t span = hi - base30;
+    const uint64_t per_block = (uint64_t)kGenBlock * kGenSpan;
+    const uint64_t blocks = (span + per_block - 1) / per_block;
+    if (blocks > 0x7FFFFFFFull) return set_error(PFC_ERR_CAPACITY, "segment too wide");
+    hipLaunchKernelGGL(k_generate, dim3((uint32_t)blocks), dim3(kGenBlock), 0, st, base30, lo, hi,
+                       d_p, cap, d_ctrl);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "k_generate launch");
+}
+
+int launch_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uint64_t* d_one,
+                 uint64_t* d_ra, uint64_t* d_rb, const pfc_setup_table* a,
+                 const pfc_setup_table* b, uint32_t cap, hipStream_t st) {
+    if (!a) return set_error(PFC_ERR_INVALID, "setup table A missing");
+    if (d_rb && !b) return set_error(PFC_ERR_INVALID, "setup table B missing");
+    uint32_t blocks = (cap + 255) / 256;
+    if (blocks == 0) blocks = 1;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, st, d_p, d_ctrl, d_v, d_one, d_ra, d_rb,
+                       a->chunks, a->groups, a->nbits, b ? b->chunks : nullptr,
+                       b ? b->groups : nullptr, b ? b->nbits : 0u, cap);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "k_setup launch");
+}
+
+template <int MODE, int K>
+static void walk_launch(const WalkArgs& A, uint32_t block, hipStream_t st) {
+    const uint32_t lanes = (A.cap + K - 1) / K;
+    const uint32_t blocks = (lanes + block - 1) / block;
+    hipLaunchKernelGGL((k_walk<MODE, K>), dim3(blocks ? blocks : 1), dim3(block), 0, st, A);
+}
+
+int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list,
+                const uint32_t* d_comp, uint32_t nbase, const uint64_t* d_p, const uint32_t* d_v,
+                const uint64_t* d_one, uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
+                pfc_factor* d_fac, uint32_t fcap, uint32_t cap, uint32_t ppl, uint32_t block,
+                hipStream_t st) {
+    if (j1 <= j0 || cap == 0) return PFC_OK;
+    WalkArgs A;
+    A.j0 = j0; A.j1 = j1; A.list = d_list; A.composite = d_comp; A.nbase = nbase;
+    A.P = d_p; A.V = d_v; A.ONE = d_one; A.RA = d_ra; A.RB = d_rb; A.ctrl = d_ctrl;
+    A.fac = d_fac; A.fcap = fcap; A.cap = cap;
+    if (block != 64 && block != 128 && block != 256) block = 256;
+    int mode;
+    if (modes == PFC_MODE_FACTORIAL) {
+        mode = 0;
+    } else if (modes == PFC_MODE_PRIMORIAL) {
+        if (!d_list) return set_error(PFC_ERR_INVALID, "walk: multiplier list missing");
+        mode = 1;
+    } else if (modes == PFC_MODE_COMPOSITORIAL) {
+        if (!d_comp) return set_error(PFC_ERR_INVALID, "walk: composite bitmap missing");
+        mode = 3;
+    } else if (modes == (PFC_MODE_FACTORIAL | PFC_MODE_COMPOSITORIAL)) {
+        if (!d_comp || !d_rb) return set_error(PFC_ERR_INVALID, "walk: composite bitmap missing");
+        mode = 2;
+    } else {
+        return set_error(PFC_ERR_INVALID, "walk: invalid mode set");
+    }
+#define PFC_WALK_CASE(MD, KK) \
+    if (mode == MD && ppl == KK) { walk_launch<MD, KK>(A, block, st); goto launched; }
+    PFC_WALK_CASE(0, 1) PFC_WALK_CASE(0, 2) PFC_WALK_CASE(0, 4)
+    PFC_WALK_CASE(1, 1) PFC_WALK_CASE(1, 2) PFC_WALK_CASE(1, 4)
+    PFC_WALK_CASE(2, 1) PFC_WALK_CASE(2, 2) PFC_WALK_CASE(2, 4)
+    PFC_WALK_CASE(3, 1) PFC_WALK_CASE(3, 2) PFC_WALK_CASE(3, 4)
+#undef PFC_WALK_CASE
+    return set_error(PFC_ERR_INVALID, "walk: primes_per_lane must be 1, 2 or 4");
+launched:
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "k_walk launch");
+}
+
+int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint32_t* d_v,
+                 const uint64_t* d_one, const uint64_t* d_ra, const uint64_t* d_rb,
+                 pfc_ctrl* d_ctrl, uint32_t cap, hipStream_t st) {
+    uint32_t blocks = (cap + 255) / 256;
+    if (blocks == 0) blocks = 1;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_check, dim3(blocks), dim3(256), 0, st, modes, nmax, d_p, d_v, d_one, d_ra,
+                       d_rb, d_ctrl, cap);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "k_check launch");
+}
+
+int probe_mul_rates(int cus, double rates[3]) {
+    const int blocks = cus * 16, iters = 2000;
+    uint32_t* d = nullptr;
+    hipError_t e = hipMalloc(&d, (size_t)blocks * 256 * 4);
+    if (e != hipSuccess) return hip_fail(e, "probe hipMalloc");
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    for (int op = 0; op < 3; ++op) {
+        auto kern = op == 0 ? k_probe<0> : op == 1 ? k_probe<1> : k_probe<2>;
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, 0, d, 0x9e3779b1u, 4);
+        (void)hipEventRecord(a, 0);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, 0, d, 0x9e3779b1u, iters);
+        (void)hipEventRecord(b, 0);
+        e = hipEventSynchronize(b);
+        if (e != hipSuccess) break;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        rates[op] = (double)blocks * 256 * iters * 16 * 8 / (ms * 1e-3);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(d);
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "probe");
+}
+
+}  // namespace pfc
+
+extern "C" {
+
+const char* pfc_last_error(void) { return pfc::g_err; }
+
+int pfc_stage_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pfc_ctrl* d_ctrl,
+                       void* stream) {
+    pfc::g_err[0] = 0;
+    if (!d_p || !d_ctrl) return pfc::set_error(PFC_ERR_INVALID, "generate: null pointer");
+    return pfc::launch_generate(lo, hi, d_p, cap, d_ctrl, (hipStream_t)stream);
+}
+
+int pfc_stage_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uint64_t* d_one,
+                    uint64_t* d_res_a, uint64_t* d_res_b, const pfc_setup_table* a,
+                    const pfc_setup_table* b, uint32_t cap, void* stream) {
+    pfc::g_err[0] = 0;
+    if (!d_p || !d_ctrl || !d_v || !d_one || !d_res_a)
+        return pfc::set_error(PFC_ERR_INVALID, "setup: null pointer");
+    return pfc::launch_setup(d_p, d_ctrl, d_v, d_one, d_res_a, d_res_b, a, b, cap,
+                             (hipStream_t)stream);
+}
+
+int pfc_stage_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list,
+                   const uint32_t* d_composite, uint32_t nbase, const uint64_t* d_p,
+                   const uint32_t* d_v, const uint64_t* d_one, uint64_t* d_res_a,
+                   uint64_t* d_res_b, pfc_ctrl* d_ctrl, pfc_factor* d_fac, uint32_t fcap,
+                   uint32_t cap, void* stream) {
+    pfc::g_err[0] = 0;
+    if (!d_p || !d_v || !d_one || !d_res_a || !d_ctrl || (!d_fac && fcap))
+        return pfc::set_error(PFC_ERR_INVALID, "walk: null pointer");
+    return pfc::launch_walk(modes, j0, j1, d_list, d_composite, nbase, d_p, d_v, d_one, d_res_a,
+                            d_res_b, d_ctrl, d_fac, fcap, cap, 4, 256, (hipStream_t)stream);
+}
+
+int pfc_stage_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint32_t* d_v,
+                    const uint64_t* d_one, const uint64_t* d_res_a, const uint64_t* d_res_b,
+                    pfc_ctrl* d_ctrl, uint32_t cap, void* stream) {
+    pfc::g_err[0] = 0;
+    if (!d_p || !d_v || !d_one || !d_res_a || !d_ctrl)
+        return pfc::set_error(PFC_ERR_INVALID, "check: null pointer");
+    if ((modes & PFC_MODE_FACTORIAL) && (modes & PFC_MODE_COMPOSITORIAL) && !d_res_b)
+        return pfc::set_error(PFC_ERR_INVALID, "check: res_b missing");
+    return pfc::launch_check(modes, nmax, d_p, d_v, d_one, d_res_a, d_res_b, d_ctrl, cap,
+                             (hipStream_t)stream);
+}
+
+}  // extern "C"

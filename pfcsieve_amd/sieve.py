@@ -1,0 +1,177 @@
+"""Python host API over libpfcsieve.so, mirroring the reference's search driver.
+
+``run_sieve`` is the counterpart of ``cl_sieve(hardware, st, sd)``
+(cl_sieve.cpp:1328-1761): it sieves one job and returns the same three
+numbers the reference prints and its self-test checks (factorcount,
+primecount, checksum; cl_sieve.cpp:1748, 1787-2080), plus the factor lines of
+``factors.txt`` (cl_sieve.cpp:565-577, 805-816).  Errors that make the
+reference exit raise ``PfcError`` with the reference's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+from ._native import Factor, JobDesc, JobStats, Tuning, check, lib
+
+MODE_FACTORIAL, MODE_PRIMORIAL, MODE_COMPOSITORIAL = 1, 2, 4
+FACTORIAL, PRIMORIAL, COMPOSITORIAL = 0, 1, 2
+
+
+class Device:
+    """One HIP device (replaces the OpenCL platform/context/queue, main.cpp:252-292)."""
+
+    def __init__(self, ordinal: int = 0):
+        h = C.c_void_p()
+        check(lib().pfc_device_open(ordinal, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        name = C.create_string_buffer(128)
+        cus = C.c_int()
+        hbm = C.c_uint64()
+        check(lib().pfc_device_info(self._h, name, 128, C.byref(cus), C.byref(hbm)))
+        return {"name": name.value.decode(), "compute_units": cus.value, "hbm_bytes": hbm.value}
+
+    def set_tuning(self, primes_per_lane=0, block_size=0, walk_chunk=0, seg_candidates=0):
+        t = Tuning(primes_per_lane, block_size, walk_chunk, seg_candidates)
+        check(lib().pfc_set_tuning(self._h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        check(lib().pfc_get_tuning(self._h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}
+
+    def probe_mul_rates(self):
+        r = (C.c_double * 3)()
+        check(lib().pfc_probe_mul_rates(self._h, C.byref(r)))
+        return {"mad_u64_u32": r[0], "mul_lo_u32": r[1], "mul_hi_u32": r[2]}
+
+    def close(self):
+        if self._h:
+            lib().pfc_device_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def job_desc(modes, pmin, pmax, nmin, nmax, factor_capacity=0) -> JobDesc:
+    return JobDesc(modes, nmin, nmax, pmin, pmax, factor_capacity)
+
+
+class Job:
+    """One sieve job on a device (workStatus of cl_sieve.h:14-18 plus its buffers)."""
+
+    def __init__(self, dev: Device, modes, pmin, pmax, nmin, nmax, factor_capacity=0):
+        self.dev = dev
+        self.desc = job_desc(modes, pmin, pmax, nmin, nmax, factor_capacity)
+        h = C.c_void_p()
+        check(lib().pfc_job_create(dev.handle, C.byref(self.desc), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def run(self, max_segments: int = 0):
+        check(lib().pfc_job_run(self._h, max_segments))
+
+    def position(self) -> int:
+        p = C.c_uint64()
+        check(lib().pfc_job_position(self._h, C.byref(p)))
+        return p.value
+
+    def seek(self, p, dev_checksum, primecount):
+        check(lib().pfc_job_seek(self._h, p, dev_checksum, primecount))
+
+    def stats(self) -> JobStats:
+        s = JobStats()
+        check(lib().pfc_job_stats_get(self._h, C.byref(s)))
+        return s
+
+    def raw_factors(self):
+        n = C.c_size_t()
+        check(lib().pfc_job_factors(self._h, None, 0, C.byref(n)))
+        arr = (Factor * max(n.value, 1))()
+        check(lib().pfc_job_factors(self._h, arr, n.value, C.byref(n)))
+        return arr, n.value
+
+    def clear_factors(self):
+        check(lib().pfc_job_clear_factors(self._h))
+
+    def destroy(self):
+        if self._h:
+            lib().pfc_job_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def footprint(dev: Optional[Device], modes, pmin, pmax, nmin, nmax) -> int:
+    d = job_desc(modes, pmin, pmax, nmin, nmax)
+    b = C.c_uint64()
+    check(lib().pfc_job_footprint(dev.handle if dev else None, C.byref(d), C.byref(b)))
+    return b.value
+
+
+def format_factor(f: Factor) -> str:
+    buf = C.create_string_buffer(96)
+    lib().pfc_format_factor(C.byref(f), buf, 96)
+    return buf.value.decode()
+
+
+def harvest(arr, n, checksum: int, verify_threads: int = 2):
+    """getResults' host half (cl_sieve.cpp:496-593): returns (checksum, kept list)."""
+    cs = C.c_uint64(checksum)
+    kept = C.c_size_t()
+    check(lib().pfc_harvest(arr, n, verify_threads, C.byref(cs), C.byref(kept)))
+    return cs.value, [arr[i] for i in range(kept.value)]
+
+
+@dataclass
+class SieveResult:
+    factorcount: int
+    primecount: int
+    checksum: int
+    lines: List[str] = field(default_factory=list)
+    factors: List[tuple] = field(default_factory=list)   # (p, nc, type), sorted
+    device_ms: float = 0.0
+    segments: int = 0
+
+    def factors_txt(self) -> str:
+        """factors.txt content (cl_sieve.cpp:565-577, 805-816)."""
+        body = "".join(self.lines)
+        if self.factorcount:
+            return body + "%016X\n" % self.checksum
+        return body + "no factors\n%016X\n" % self.checksum
+
+
+def run_sieve(modes, pmin, pmax, nmin, nmax, *, device: Optional[Device] = None,
+              verify_threads: int = 2, factor_capacity: int = 0) -> SieveResult:
+    dev = device or Device(0)
+    job = Job(dev, modes, pmin, pmax, nmin, nmax, factor_capacity)
+    try:
+        job.run()
+        st = job.stats()
+        arr, n = job.raw_factors()
+        checksum, kept = harvest(arr, n, st.dev_checksum, verify_threads)
+        res = SieveResult(factorcount=len(kept), primecount=st.primecount, checksum=checksum,
+                          device_ms=st.device_ms, segments=st.segments)
+        res.lines = [format_factor(f) for f in kept]
+        res.factors = [(f.p, f.nc, f.type) for f in kept]
+        return res
+    finally:
+        job.destroy()
