@@ -48,6 +48,48 @@ def cpu_baseline(modes, p0, width, nmin, nmax, steps_per_cand):
                       f"{dt:.1f} s"}
 
 
+def mix_compare(dev, units, verify_threads=16):
+    """Plain (one job at a time, one range per launch sequence) against batched
+    (all units through one batcher, shared persistent launches) on the same
+    seeded unit mix.  Returns updates/s of both and the ratio."""
+    import pfcsieve_amd as P
+    from pfcsieve_amd import workload as W
+    steps = [W.walk_steps(u) for u in units]
+    # plain
+    t = time.perf_counter()
+    cand_plain, sig_plain = [], []
+    for u in units:
+        j = P.Job(dev, *u.desc())
+        j.run()
+        st = j.stats()
+        cand_plain.append(st.primecount)
+        sig_plain.append((st.dev_checksum, st.primecount, st.raw_factors))
+        j.destroy()
+    t_plain = time.perf_counter() - t
+    # batched
+    jobs = [P.Job(dev, *u.desc()) for u in units]
+    b = P.Batcher(dev)
+    t = time.perf_counter()
+    for j in jobs:
+        b.add(j)
+    rounds = 0
+    while b.pending():
+        b.round(0)
+        rounds += 1
+    t_batch = time.perf_counter() - t
+    cand_batch = [j.stats().primecount for j in jobs]
+    sig_batch = [(j.stats().dev_checksum, j.stats().primecount, j.stats().raw_factors) for j in jobs]
+    for j in jobs:
+        j.destroy()
+    b.destroy()
+    assert cand_plain == cand_batch
+    upd = sum(c * s for c, s in zip(cand_plain, steps))
+    return {"units": len(units), "candidates": sum(cand_plain), "walk_updates": upd,
+            "plain_s": t_plain, "batched_s": t_batch, "plain_updates_per_s": upd / t_plain,
+            "batched_updates_per_s": upd / t_batch, "batched_over_plain": t_plain / t_batch,
+            "rounds": rounds, "results_identical": sig_plain == sig_batch}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +100,9 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-width", type=int, default=150000)
+    ap.add_argument("--mix-units", type=int, default=24,
+                    help="units of the seeded 256-unit batched mix to compare plain vs batched (0 = skip)")
+    ap.add_argument("--autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding over p is "
@@ -74,6 +119,8 @@ def main():
     rates = dev.probe_mul_rates()
     p_mul = max(rates.values())
     dev.set_tuning(seg_candidates=args.seg_candidates, primes_per_lane=args.ppl, block_size=args.block)
+    if args.autotune:
+        dev.autotune(modes, pmin, pmin + 2 * 10**9, nmin, nmax)
     total = args.steps + args.warmup
     # p-range: the headline 2e9 window, or wider if more segments are asked for
     job = P.Job(dev, modes, pmin, pmin + 2 * 10**9 * max(1, (total + 16) // 17), nmin, nmax)
@@ -107,6 +154,13 @@ def main():
                              f"(mad_u64_u32 {rates['mad_u64_u32']:.3e}, mul_lo {rates['mul_lo_u32']:.3e}, "
                              f"mul_hi {rates['mul_hi_u32']:.3e})"},
     }
+    if args.mix_units:
+        from pfcsieve_amd import workload as W
+        units = W.mix(256, seed=12345)[: args.mix_units]
+        line["batched"] = mix_compare(dev, units)
+        line["batched"]["mix"] = ("first %d of 256 seeded units (splitmix64 seed 12345): modes F/P/C equal odds, "
+                                  "n-width log-U[1e3,1e6] from 101+U[0,1e6], p-width log-U[1e6,1e9] from "
+                                  "log-U[1e10,2^63]" % args.mix_units)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(modes, pmin, args.cpu_width, nmin, nmax, steps_per_cand)
         line["cpu_baseline"]["gpu_over_cpu"] = ups / line["cpu_baseline"]["value"]

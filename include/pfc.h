@@ -98,6 +98,11 @@ int pfc_device_info(const pfc_device* dev, char* name, size_t name_cap, int* com
                     uint64_t* hbm_bytes);
 int pfc_set_tuning(pfc_device* dev, const pfc_tuning* t);
 int pfc_get_tuning(const pfc_device* dev, pfc_tuning* t);
+/* Project autotuner: times the walk kernel at every primes_per_lane x
+ * block_size shape on a sample of `desc` and stores the fastest in the
+ * device tuning (also returned in *best).  Counterpart of the reference's
+ * profile-driven sizing (cl_sieve.cpp:669-769, 1642-1694). */
+int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tuning* best);
 
 /* ------------------------------------------------------------------ jobs */
 /* Validates like setupSearch (cl_sieve.cpp:596-665) and reports the HBM bytes
@@ -122,9 +127,27 @@ int pfc_job_factors(const pfc_job* job, pfc_factor* out, size_t cap, size_t* n);
 int pfc_job_clear_factors(pfc_job* job);
 int pfc_job_destroy(pfc_job* job);
 
-/* Runs several jobs as one batch: their segments are packed into shared
- * launches of a persistent walk kernel fed by a device work queue
- * (no counterpart in the reference, which runs one range per launch). */
+/* ------------------------------------------------------------------ batching */
+/* No counterpart in the reference, which runs one p-range per launch
+ * sequence.  A batcher runs many jobs together: each round generates new
+ * p-segments for jobs that need one, cuts every active segment into work
+ * items (64*K candidates x up to max_steps multiplier steps, longest first)
+ * and drains them with ONE launch of a persistent kernel fed by a device work
+ * queue (set-up, walk and checksum fold fused).  Job state stays resident on
+ * the device between rounds; jobs can be added between rounds (continuous
+ * batching).  Results land in the jobs exactly as pfc_job_run leaves them. */
+typedef struct pfc_batcher pfc_batcher;
+int pfc_batcher_create(pfc_device* dev, pfc_batcher** out);
+/* The job stays owned by the caller and must outlive its time in the batcher. */
+int pfc_batcher_add(pfc_batcher* b, pfc_job* job);
+/* Jobs still in the batcher. */
+int pfc_batcher_pending(const pfc_batcher* b, size_t* n);
+/* One round (blocking).  max_steps == 0: walk each segment to its end in one
+ * round.  Jobs that finished in this round are written to done[0..cap) and
+ * counted in *ndone, and leave the batcher. */
+int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** done, size_t cap, size_t* ndone);
+int pfc_batcher_destroy(pfc_batcher* b);
+/* Runs jobs to completion through one batcher. */
 int pfc_batch_run(pfc_job** jobs, size_t njobs);
 
 /* ------------------------------------------------------------------ harvest */

@@ -62,6 +62,7 @@ SIGNATURES = {
     "pfc_device_info": (_int, [_vp, C.c_char_p, _sz, C.POINTER(_int), C.POINTER(_u64)]),
     "pfc_set_tuning": (_int, [_vp, C.POINTER(Tuning)]),
     "pfc_get_tuning": (_int, [_vp, C.POINTER(Tuning)]),
+    "pfc_autotune": (_int, [_vp, C.POINTER(JobDesc), C.POINTER(Tuning)]),
     "pfc_job_footprint": (_int, [_vp, C.POINTER(JobDesc), C.POINTER(_u64)]),
     "pfc_job_create": (_int, [_vp, C.POINTER(JobDesc), C.POINTER(_vp)]),
     "pfc_job_run": (_int, [_vp, _u64]),
@@ -72,6 +73,11 @@ SIGNATURES = {
     "pfc_job_clear_factors": (_int, [_vp]),
     "pfc_job_destroy": (_int, [_vp]),
     "pfc_batch_run": (_int, [C.POINTER(_vp), _sz]),
+    "pfc_batcher_create": (_int, [_vp, C.POINTER(_vp)]),
+    "pfc_batcher_add": (_int, [_vp, _vp]),
+    "pfc_batcher_pending": (_int, [_vp, C.POINTER(_sz)]),
+    "pfc_batcher_round": (_int, [_vp, _u32, C.POINTER(_vp), _sz, C.POINTER(_sz)]),
+    "pfc_batcher_destroy": (_int, [_vp]),
     "pfc_harvest": (_int, [C.POINTER(Factor), _sz, _int, C.POINTER(_u64), C.POINTER(_sz)]),
     "pfc_format_factor": (_int, [C.POINTER(Factor), C.c_char_p, _sz]),
     "pfc_is_prime": (_int, [_u64]),

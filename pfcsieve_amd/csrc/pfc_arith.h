@@ -66,11 +66,34 @@ PFC_HD uint64_t div3by2_rem(uint32_t u2, uint32_t u1, uint32_t u0, const Modulus
     return r;
 }
 
+// (a * n) mod d for a < d and any 32-bit n, without the final (rare)
+// correction: returns x with x == (a*n mod d) or x == (a*n mod d) + d, and
+// sets *fix when the caller must subtract d (x >= d).  The walk kernel uses
+// this form so that the rare correction becomes a wavefront-uniform branch.
+PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t d1, uint32_t d0, uint64_t d,
+                               uint32_t v, bool* fix) {
+    const uint64_t lo = (uint64_t)(uint32_t)a * n;
+    const uint64_t hi = (uint64_t)(uint32_t)(a >> 32) * n + (lo >> 32);   // <u2,u1>
+    const uint32_t u0 = (uint32_t)lo, u1 = (uint32_t)hi, u2 = (uint32_t)(hi >> 32);
+    const uint64_t qq = (uint64_t)v * u2 + hi;
+    const uint32_t q1 = (uint32_t)(qq >> 32), q0 = (uint32_t)qq;
+    const uint64_t t = (uint64_t)d0 * q1 + d;
+    const uint32_t r1 = u1 - q1 * d1;
+    const uint32_t t0 = (uint32_t)t;
+    const uint32_t xlo = u0 - t0;
+    const uint32_t xhi = r1 - (uint32_t)(t >> 32) - (uint32_t)(u0 < t0);
+    uint64_t x = ((uint64_t)xhi << 32) | xlo;
+    if (xhi >= q0) x += d;
+    *fix = x >= d;
+    return x;
+}
+
 // (a * n) mod d for a < d and any 32-bit n.  Keeps the shifted form.
 PFC_HD uint64_t mulmod32(uint64_t a, uint32_t n, const Modulus& m) {
-    const uint64_t lo = (uint64_t)(uint32_t)a * n;
-    const uint64_t hi = (uint64_t)(uint32_t)(a >> 32) * n + (lo >> 32);
-    return div3by2_rem((uint32_t)(hi >> 32), (uint32_t)hi, (uint32_t)lo, m);
+    bool fix;
+    const uint64_t d = mod_d(m);
+    const uint64_t x = mulmod32_nofix(a, n, m.d1, m.d0, d, m.v, &fix);
+    return fix ? x - d : x;
 }
 
 // (a * 2^32) mod d for a < d.
@@ -163,6 +186,52 @@ PFC_HD bool strong_prp2(uint64_t N) {
         if (a == nmo) return true;
     }
     return false;
+}
+
+// ---------------------------------------------------------------- wheel + small-prime sieve
+
+// The sieve primes of the reference's candidate generator (kernels/
+// getsegprimes.cl:488-491): every prime from 7 to 113.
+constexpr uint32_t kSievePrimes[27] = {7,  11, 13, 17, 19, 23, 29, 31,  37,  41,  43,  47,  53, 59,
+                                       61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113};
+
+// Bit j set iff q divides j, for 0 <= j < 64.
+PFC_HD constexpr uint64_t sieve_pattern(uint32_t q) {
+    uint64_t m = 0;
+    for (uint32_t j = 0; j < 64; j += q) m |= 1ull << j;
+    return m;
+}
+
+// Offsets 0 <= j < 60 with start + j coprime to 30, for start a multiple of 30.
+constexpr uint64_t kWheel60 = (1ull << 1) | (1ull << 7) | (1ull << 11) | (1ull << 13) |
+                              (1ull << 17) | (1ull << 19) | (1ull << 23) | (1ull << 29) |
+                              (1ull << 31) | (1ull << 37) | (1ull << 41) | (1ull << 43) |
+                              (1ull << 47) | (1ull << 49) | (1ull << 53) | (1ull << 59);
+
+// Survivors of the 60-wide window [start, start + 60), start a multiple of
+// 30: bit j is set iff lo <= start + j < hi, start + j is coprime to 30 and
+// has no prime factor from 7 to 113.  rmod[i] = start mod kSievePrimes[i].
+PFC_HD uint64_t window_survivors(uint64_t start, uint64_t lo, uint64_t hi, const uint32_t* rmod) {
+    uint64_t cand = kWheel60;
+    if (start < lo) {
+        const uint64_t skip = lo - start;
+        cand = skip >= 64 ? 0 : cand & (~0ull << skip);
+    }
+    if (hi <= start) {
+        cand = 0;
+    } else if (hi - start < 64) {
+        cand &= (1ull << (hi - start)) - 1;
+    }
+    if (!cand) return 0;
+    uint64_t hit = 0;
+    for (int i = 0; i < 27; ++i) {
+        const uint32_t q = kSievePrimes[i];
+        const uint32_t j0 = rmod[i] ? q - rmod[i] : 0;   // offset of the first multiple of q
+        // A shift count of 64 or more is out of range (the hardware would use
+        // it mod 64), and such a multiple lies outside the window anyway.
+        if (j0 < 64) hit |= sieve_pattern(q) << j0;
+    }
+    return cand & ~hit;
 }
 
 }  // namespace pfc

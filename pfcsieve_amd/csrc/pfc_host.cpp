@@ -581,12 +581,364 @@ extern "C" int pfc_job_destroy(pfc_job* j) {
     return PFC_OK;
 }
 
-extern "C" int pfc_batch_run(pfc_job** jobs, size_t njobs) {
-    for (size_t i = 0; i < njobs; ++i) {
-        int rc = pfc_job_run(jobs[i], 0);
+// Times walk launches of every (primes_per_lane, block_size) shape on a
+// sample of the job (a p-window with about 2^20 candidates, generated and set
+// up once) and keeps the fastest in the device tuning.  The counterpart of
+// the reference's profile-driven sizing (cl_sieve.cpp:669-769, 1642-1694).
+extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tuning* best) {
+    pfc::g_err[0] = 0;
+    if (!dev || !desc) return set_error(PFC_ERR_INVALID, "autotune: null pointer");
+    int rc = validate(desc);
+    if (rc) return rc;
+    pfc_job_desc d = *desc;
+    const double lnp = log((double)std::max<uint64_t>(d.pmin, 16));
+    const uint64_t want = (uint64_t)((double)(1u << 20) * lnp);
+    if (d.pmax - d.pmin > want) d.pmax = d.pmin + want;
+    pfc_tuning saved = dev->tuning;
+    pfc_tuning t = saved;
+    t.seg_candidates = 1u << 20;
+    {
+        std::lock_guard<std::mutex> g(dev->mu);
+        dev->tuning = t;
+    }
+    pfc_job* j = nullptr;
+    rc = pfc_job_create(dev, &d, &j);
+    if (rc) {
+        dev->tuning = saved;
+        return rc;
+    }
+    hipStream_t st = j->stream;
+    const uint32_t cap = j->plan.cap;
+    hipError_t e = hipMemsetAsync(j->d_ctrl, 0, sizeof(pfc_ctrl), st);
+    if (e != hipSuccess) rc = hip_fail(e, "autotune clear");
+    if (!rc) rc = pfc::launch_generate(d.pmin, d.pmin + j->plan.span, j->d_p, cap, j->d_ctrl, st);
+    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits};
+    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits};
+    if (!rc)
+        rc = pfc::launch_setup(j->d_p, j->d_ctrl, j->d_v, j->d_one, j->d_ra, j->d_rb, &ta,
+                               j->d_rb ? &tb : nullptr, cap, st);
+    const uint32_t steps = std::min<uint32_t>(4096, j->jend - j->jbegin);
+    float best_ms = 1e30f;
+    pfc_tuning win = saved;
+    const uint32_t ppls[3] = {1, 2, 4}, blocks[3] = {64, 128, 256};
+    for (int rep = 0; rep < 2 && !rc; ++rep) {      // first pass warms up
+        for (uint32_t ppl : ppls) {
+            for (uint32_t blk : blocks) {
+                if (rc) break;
+                (void)hipEventRecord(j->ev0, st);
+                rc = pfc::launch_walk(d.modes, j->jbegin, j->jbegin + steps, j->d_list, j->d_comp, d.nmin,
+                                      j->d_p, j->d_v, j->d_one, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac,
+                                      (uint32_t)j->fcap, cap, ppl, blk, st);
+                (void)hipEventRecord(j->ev1, st);
+                e = hipEventSynchronize(j->ev1);
+                if (e != hipSuccess) rc = hip_fail(e, "autotune walk");
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, j->ev0, j->ev1);
+                if (rep == 1 && ms < best_ms) {
+                    best_ms = ms;
+                    win.primes_per_lane = ppl;
+                    win.block_size = blk;
+                }
+            }
+        }
+    }
+    pfc_job_destroy(j);
+    {
+        std::lock_guard<std::mutex> g(dev->mu);
+        dev->tuning = rc ? saved : win;
+    }
+    if (best) *best = dev->tuning;
+    return rc;
+}
+
+// ====================================================================== batcher
+//
+// Continuous batching of many jobs on one device.  Every round:
+//   1. jobs that need a new p-segment get one (clear control block, generate
+//      candidates) -- all jobs' generation launches are queued back to back;
+//   2. the candidates of every active segment are cut into work items of
+//      64*K candidates x up to `max_steps` multiplier steps, ordered by
+//      decreasing cost (longest first), and ONE launch of the persistent
+//      k_batch kernel drains them from a device queue;
+//   3. segments whose walk finished (their last items carried the checksum
+//      fold) are drained to the host like pfc_job_run does; finished jobs are
+//      returned to the caller.
+// Jobs can be added between rounds; their state stays resident on the device.
+
+struct pfc_batcher {
+    pfc_device* dev = nullptr;
+    hipStream_t stream = nullptr;
+    struct Slot {
+        pfc_job* job;
+        int stage;          // 0 = needs a segment, 1 = walking, 2 = done
+        uint64_t seg_hi;    // end of the current segment
+        uint64_t span;      // current segment width
+        uint32_t jcur;      // next multiplier step of the current segment
+        uint32_t ncand;
+    };
+    std::vector<Slot> slots;
+    pfc_batch_job* d_jobs = nullptr;
+    size_t d_jobs_cap = 0;
+    pfc_batch_item* d_items = nullptr;
+    size_t d_items_cap = 0;
+    uint32_t* d_head = nullptr;
+    pfc_ctrl* h_ctrl = nullptr;       // pinned, one per slot
+    size_t h_ctrl_cap = 0;
+    uint64_t rounds = 0;
+};
+
+namespace {
+
+pfc_batch_job batch_view(pfc_job* j) {
+    pfc_batch_job b{};
+    b.modes = j->desc.modes;
+    b.nmin = j->desc.nmin;
+    b.nmax = j->desc.nmax;
+    b.cap = j->plan.cap;
+    b.P = j->d_p;
+    b.V = j->d_v;
+    b.ONE = j->d_one;
+    b.RA = j->d_ra;
+    b.RB = j->d_rb;
+    b.ctrl = j->d_ctrl;
+    b.fac = j->d_fac;
+    b.fcap = (uint32_t)j->fcap;
+    b.nba = j->ta.nbits;
+    b.nbb = j->tb.nbits;
+    b.ca = j->d_ca;
+    b.ga = j->d_ga;
+    b.cb = j->d_cb;
+    b.gb = j->d_gb;
+    b.list = j->d_list;
+    b.composite = j->d_comp;
+    return b;
+}
+
+template <class T>
+int ensure_device(T** p, size_t* cap, size_t need) {
+    if (need <= *cap) return PFC_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    size_t c = std::max<size_t>(need, 2 * *cap);
+    hipError_t e = hipMalloc((void**)p, c * sizeof(T));
+    if (e != hipSuccess) {
+        *cap = 0;
+        return hip_fail(e, "batcher allocation");
+    }
+    *cap = c;
+    return PFC_OK;
+}
+
+// Drains a finished segment into the job (factor records, sums, position).
+int finish_segment(pfc_job* j, const pfc_ctrl& c, uint64_t seg_hi) {
+    if (c.flags & 2u) return set_error(PFC_ERR_VALIDATION, "error: gpu validation failure");
+    if (c.nfactor > j->fcap) {
+        char msg[128];
+        snprintf(msg, sizeof(msg), "Error: number of results (%u) overflowed array.", c.nfactor);
+        return set_error(PFC_ERR_FACTOR_OVERFLOW, msg);
+    }
+    if (c.nfactor) {
+        const size_t old = j->factors.size();
+        j->factors.resize(old + c.nfactor);
+        hipError_t e = hipMemcpy(j->factors.data() + old, j->d_fac, (size_t)c.nfactor * sizeof(pfc_factor),
+                                 hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "factor copy");
+    }
+    j->dev_checksum += c.checksum;
+    j->primecount += c.primecount;
+    j->segments += 1;
+    j->pos = seg_hi;
+    return PFC_OK;
+}
+
+}  // namespace
+
+extern "C" int pfc_batcher_create(pfc_device* dev, pfc_batcher** out) {
+    pfc::g_err[0] = 0;
+    if (!dev || !out) return set_error(PFC_ERR_INVALID, "batcher_create: null pointer");
+    hipSetDevice(dev->ordinal);
+    pfc_batcher* b = new pfc_batcher();
+    b->dev = dev;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_head, sizeof(uint32_t));
+    if (e != hipSuccess) {
+        int rc = hip_fail(e, "batcher create");
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return PFC_OK;
+}
+
+extern "C" int pfc_batcher_add(pfc_batcher* b, pfc_job* j) {
+    if (!b || !j) return set_error(PFC_ERR_INVALID, "batcher_add: null pointer");
+    if (j->dev != b->dev) return set_error(PFC_ERR_INVALID, "batcher_add: job on another device");
+    pfc_batcher::Slot s{j, j->pos >= j->desc.pmax ? 2 : 0, 0, j->plan.span, 0, 0};
+    b->slots.push_back(s);
+    return PFC_OK;
+}
+
+extern "C" int pfc_batcher_pending(const pfc_batcher* b, size_t* n) {
+    if (!b || !n) return set_error(PFC_ERR_INVALID, "batcher_pending: null pointer");
+    *n = b->slots.size();
+    return PFC_OK;
+}
+
+extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** done, size_t cap,
+                                 size_t* ndone) {
+    pfc::g_err[0] = 0;
+    if (!b || !ndone) return set_error(PFC_ERR_INVALID, "batcher_round: null pointer");
+    *ndone = 0;
+    hipSetDevice(b->dev->ordinal);
+    hipStream_t st = b->stream;
+    if (max_steps == 0) max_steps = 0xFFFFFFFFu;
+    const size_t ns = b->slots.size();
+    int rc = PFC_OK;
+    if (b->h_ctrl_cap < ns) {
+        if (b->h_ctrl) (void)hipHostFree(b->h_ctrl);
+        b->h_ctrl = nullptr;
+        b->h_ctrl_cap = std::max<size_t>(ns, 2 * b->h_ctrl_cap);
+        hipError_t e = hipHostMalloc((void**)&b->h_ctrl, b->h_ctrl_cap * sizeof(pfc_ctrl), hipHostMallocDefault);
+        if (e != hipSuccess) return hip_fail(e, "batcher pinned ctrl");
+    }
+    // 1. new segments
+    bool any_gen = false;
+    for (size_t i = 0; i < ns; ++i) {
+        pfc_batcher::Slot& s = b->slots[i];
+        if (s.stage != 0) continue;
+        pfc_job* j = s.job;
+        uint64_t hi = j->pos + s.span;
+        if (hi > j->desc.pmax || hi < j->pos) hi = j->desc.pmax;
+        s.seg_hi = hi;
+        hipError_t e = hipMemsetAsync(j->d_ctrl, 0, sizeof(pfc_ctrl), st);
+        if (e != hipSuccess) return hip_fail(e, "batcher ctrl clear");
+        rc = pfc::launch_generate(j->pos, hi, j->d_p, j->plan.cap, j->d_ctrl, st);
+        if (rc) return rc;
+        e = hipMemcpyAsync(&b->h_ctrl[i], j->d_ctrl, sizeof(pfc_ctrl), hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) return hip_fail(e, "batcher ctrl read");
+        any_gen = true;
+    }
+    if (any_gen) {
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "batcher generate");
+        for (size_t i = 0; i < ns; ++i) {
+            pfc_batcher::Slot& s = b->slots[i];
+            if (s.stage != 0) continue;
+            const pfc_ctrl c = b->h_ctrl[i];
+            if (c.flags & 1u) {  // more candidates than the bound: halve and retry next round
+                if (s.span <= 1) return set_error(PFC_ERR_CAPACITY, "candidate buffer overflow");
+                s.span /= 2;
+                continue;
+            }
+            s.ncand = c.ncand;
+            s.jcur = s.job->jbegin;
+            s.stage = 1;
+        }
+    }
+    // 2. work items
+    std::vector<pfc_batch_job> jobs(ns);
+    struct Tmp { pfc_batch_item it; double cost; };
+    std::vector<Tmp> items;
+    const uint32_t K = b->dev->tuning.primes_per_lane;
+    std::vector<uint8_t> finishing(ns, 0);
+    for (size_t i = 0; i < ns; ++i) {
+        pfc_batcher::Slot& s = b->slots[i];
+        jobs[i] = batch_view(s.job);
+        if (s.stage != 1) continue;
+        pfc_job* j = s.job;
+        const uint32_t j0 = s.jcur;
+        const uint32_t j1 = (uint32_t)std::min<uint64_t>(j->jend, (uint64_t)j0 + max_steps);
+        uint32_t flags = (j0 == j->jbegin ? 1u : 0u) | (j1 == j->jend ? 2u : 0u);
+        if (s.ncand == 0) {
+            finishing[i] = 1;  // nothing to walk; the segment is complete
+            s.jcur = j->jend;
+            continue;
+        }
+        const double per = (double)(j1 - j0) * ((j->d_rb) ? 1.9 : 1.0) +
+                           ((flags & 1u) ? 1.3 * (double)(j->ta.chunks.size() + j->tb.chunks.size()) : 0.0);
+        for (uint32_t f = 0; f < s.ncand; f += 64 * K) items.push_back({{(uint32_t)i, f, j0, j1, flags, {0, 0, 0}}, per});
+        s.jcur = j1;
+        if (j1 == j->jend) finishing[i] = 1;
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Tmp& a, const Tmp& c) { return a.cost > c.cost; });
+    if (!items.empty()) {
+        rc = ensure_device(&b->d_jobs, &b->d_jobs_cap, ns);
+        if (!rc) rc = ensure_device(&b->d_items, &b->d_items_cap, items.size());
+        if (rc) return rc;
+        std::vector<pfc_batch_item> flat(items.size());
+        for (size_t k = 0; k < items.size(); ++k) flat[k] = items[k].it;
+        hipError_t e = hipMemcpyAsync(b->d_jobs, jobs.data(), ns * sizeof(pfc_batch_job), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->d_items, flat.data(), flat.size() * sizeof(pfc_batch_item), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemsetAsync(b->d_head, 0, sizeof(uint32_t), st);
+        if (e != hipSuccess) return hip_fail(e, "batcher upload");
+        const uint32_t nblocks = (uint32_t)std::min<size_t>(items.size(), (size_t)b->dev->prop.multiProcessorCount * 32);
+        rc = pfc::launch_batch(b->d_jobs, b->d_items, (uint32_t)items.size(), b->d_head, K, nblocks, st);
         if (rc) return rc;
     }
+    // 3. drain finished segments
+    bool any_fin = false;
+    for (size_t i = 0; i < ns; ++i) {
+        if (!finishing[i]) continue;
+        hipError_t e = hipMemcpyAsync(&b->h_ctrl[i], b->slots[i].job->d_ctrl, sizeof(pfc_ctrl),
+                                      hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) return hip_fail(e, "batcher ctrl read");
+        any_fin = true;
+    }
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "batcher round");
+    b->rounds += 1;
+    if (any_fin) {
+        std::vector<pfc_batcher::Slot> keep;
+        for (size_t i = 0; i < ns; ++i) {
+            pfc_batcher::Slot s = b->slots[i];
+            if (finishing[i]) {
+                rc = finish_segment(s.job, b->h_ctrl[i], s.seg_hi);
+                if (rc) return rc;
+                s.stage = s.job->pos >= s.job->desc.pmax ? 2 : 0;
+            }
+            if (s.stage == 2) {
+                if (done && *ndone < cap) done[*ndone] = s.job;
+                ++*ndone;
+            } else {
+                keep.push_back(s);
+            }
+        }
+        b->slots.swap(keep);
+    }
     return PFC_OK;
+}
+
+extern "C" int pfc_batcher_destroy(pfc_batcher* b) {
+    if (!b) return PFC_OK;
+    hipSetDevice(b->dev->ordinal);
+    (void)hipStreamSynchronize(b->stream);
+    if (b->d_jobs) (void)hipFree(b->d_jobs);
+    if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_head) (void)hipFree(b->d_head);
+    if (b->h_ctrl) (void)hipHostFree(b->h_ctrl);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return PFC_OK;
+}
+
+extern "C" int pfc_batch_run(pfc_job** jobs, size_t njobs) {
+    pfc::g_err[0] = 0;
+    if (!jobs && njobs) return set_error(PFC_ERR_INVALID, "batch_run: null jobs");
+    if (njobs == 0) return PFC_OK;
+    pfc_batcher* b = nullptr;
+    int rc = pfc_batcher_create(jobs[0]->dev, &b);
+    if (rc) return rc;
+    for (size_t i = 0; i < njobs && !rc; ++i) rc = pfc_batcher_add(b, jobs[i]);
+    size_t left = njobs;
+    while (!rc && left) {
+        size_t nd = 0;
+        rc = pfc_batcher_round(b, 0, nullptr, 0, &nd);
+        left -= std::min(left, nd);
+    }
+    pfc_batcher_destroy(b);
+    return rc;
 }
 
 // ====================================================================== harvest

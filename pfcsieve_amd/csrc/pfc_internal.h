@@ -9,6 +9,31 @@
 
 #include "../../include/pfc.h"
 
+// One job's device state as the persistent batch kernel sees it.
+struct pfc_batch_job {
+    uint32_t modes, nmin, nmax, cap;
+    uint64_t* P;
+    uint32_t* V;
+    uint64_t* ONE;
+    uint64_t* RA;
+    uint64_t* RB;
+    pfc_ctrl* ctrl;
+    pfc_factor* fac;
+    uint32_t fcap, nba, nbb, pad;
+    const uint32_t* ca;
+    const uint32_t* ga;
+    const uint32_t* cb;
+    const uint32_t* gb;
+    const uint32_t* list;
+    const uint32_t* composite;
+};
+
+// One work item: 64*K consecutive candidates of a job from index `first`,
+// multiplier steps [j0, j1); flags bit0 = set up first, bit1 = fold checksum.
+struct pfc_batch_item {
+    uint32_t job, first, j0, j1, flags, pad[3];
+};
+
 namespace pfc {
 
 extern thread_local char g_err[512];
@@ -28,6 +53,8 @@ int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list
 int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint32_t* d_v,
                  const uint64_t* d_one, const uint64_t* d_ra, const uint64_t* d_rb,
                  pfc_ctrl* d_ctrl, uint32_t cap, hipStream_t st);
+int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_item* d_items, uint32_t nitems,
+                 uint32_t* d_head, uint32_t ppl, uint32_t nblocks, hipStream_t st);
 int probe_mul_rates(int cus, double rates[3]);
 
 }  // namespace pfc

@@ -27,36 +27,6 @@ namespace {
 
 // ------------------------------------------------------------------ generate
 
-// Bit j of kPattern[i] is set iff sieve prime i divides j (j < 64): shifted
-// left by the first multiple's offset it marks every multiple in a 64-wide
-// window of consecutive integers.
-struct SieveTables {
-    uint32_t prime[27];
-    uint64_t pattern[27];
-};
-
-constexpr SieveTables make_sieve_tables() {
-    SieveTables t{};
-    const uint32_t ps[27] = {7,  11, 13, 17, 19, 23, 29, 31,  37,  41,  43,  47,  53, 59,
-                             61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113};
-    for (int i = 0; i < 27; ++i) {
-        t.prime[i] = ps[i];
-        uint64_t m = 0;
-        for (uint32_t j = 0; j < 64; j += ps[i]) m |= 1ull << j;
-        t.pattern[i] = m;
-    }
-    return t;
-}
-
-__constant__ SieveTables c_sieve = make_sieve_tables();
-
-// Mask of the integers coprime to 30 in a 60-wide window starting at a
-// multiple of 30: offsets 1,7,11,13,17,19,23,29 and the same +30.
-constexpr uint64_t kWheel60 = (1ull << 1) | (1ull << 7) | (1ull << 11) | (1ull << 13) |
-                              (1ull << 17) | (1ull << 19) | (1ull << 23) | (1ull << 29) |
-                              (1ull << 31) | (1ull << 37) | (1ull << 41) | (1ull << 43) |
-                              (1ull << 47) | (1ull << 49) | (1ull << 53) | (1ull << 59);
-
 constexpr int kGenBlock = 256;
 constexpr int kGenSpan = 60;                      // integers per thread
 constexpr int kGenLds = kGenBlock * 16;           // at most 16 survivors per thread
@@ -92,33 +62,16 @@ __global__ __launch_bounds__(kGenBlock) void k_generate(uint64_t base30, uint64_
     __shared__ uint32_t s_count;
     const uint32_t tid = threadIdx.x;
     const uint64_t block_base = base30 + (uint64_t)blockIdx.x * (kGenBlock * kGenSpan);
-    if (tid < 27) s_mod[tid] = (uint32_t)(block_base % c_sieve.prime[tid]);
+    if (tid < 27) s_mod[tid] = (uint32_t)(block_base % pfc::kSievePrimes[tid]);
     if (tid == 0) s_count = 0;
     __syncthreads();
 
     const uint64_t start = block_base + (uint64_t)tid * kGenSpan;
-    uint64_t cand = kWheel60;
-    // Range clip: keep offsets j with lo <= start + j < hi.
-    if (start < lo) {
-        const uint64_t skip = lo - start;
-        cand = skip >= 64 ? 0 : cand & (~0ull << skip);
-    }
-    if (hi <= start) {
-        cand = 0;
-    } else if (hi - start < 64) {
-        cand &= (1ull << (hi - start)) - 1;
-    }
-    if (cand) {
-        uint64_t hit = 0;
-        const uint32_t toff = tid * kGenSpan;
+    uint32_t rmod[27];
 #pragma unroll
-        for (int i = 0; i < 27; ++i) {
-            const uint32_t q = c_sieve.prime[i];
-            const uint32_t r = (s_mod[i] + toff) % q;         // start mod q
-            const uint32_t j0 = r ? q - r : 0;                  // first multiple's offset
-            hit |= c_sieve.pattern[i] << j0;
-        }
-        cand &= ~hit;
+    for (int i = 0; i < 27; ++i) rmod[i] = (s_mod[i] + tid * kGenSpan) % pfc::kSievePrimes[i];
+    uint64_t cand = pfc::window_survivors(start, lo, hi, rmod);
+    if (cand) {
         while (cand) {
             const uint32_t j = __ffsll((unsigned long long)cand) - 1;
             cand &= cand - 1;
@@ -156,14 +109,17 @@ __global__ void k_small_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint32_t
 // Start residue of one table: Horner over exponent bits, high bit first,
 // acc = acc^2 * prod(chunks of group b), starting from 1; returned multiplied
 // by R mod p so that it is the Montgomery form (X*R mod p) << s.
-__device__ __forceinline__ uint64_t horner_start(const uint32_t* __restrict__ chunks,
-                                                 const uint32_t* __restrict__ groups,
+// The tables are uniform across the wavefront and are read with scalar loads.
+typedef const uint32_t __attribute__((address_space(4)))* ConstU32;
+
+__device__ __forceinline__ uint64_t horner_start(const uint32_t* chunks_, const uint32_t* groups_,
                                                  uint32_t nbits, const Modulus& m, uint64_t one) {
+    const ConstU32 chunks = (ConstU32)chunks_, groups = (ConstU32)groups_;
     uint64_t acc = 1ull << m.s;
     for (int b = (int)nbits - 1; b >= 0; --b) {
         if (b != (int)nbits - 1) acc = pfc::sqrmod(acc, m);
         const uint32_t g0 = groups[b], g1 = groups[b + 1];
-        for (uint32_t k = g0; k < g1; ++k) acc = pfc::mulmod32(acc, chunks[k], m);
+        for (uint32_t k = g0; k < g1; ++k) acc = pfc::mulmod32(acc, __builtin_amdgcn_readfirstlane(chunks[k]), m);
     }
     return pfc::mulmod64(acc, one >> m.s, m);
 }
@@ -220,11 +176,114 @@ struct WalkArgs {
     uint32_t cap;
 };
 
-// One lane owns K candidates (indices lane*K .. lane*K+K-1) and keeps their
-// residues in VGPRs for the whole launch.  The multiplier stream is uniform
-// across the wavefront.  MODE: 0 = F (n = j), 1 = P (n = list[j]),
-// 2 = F|C (n = j; composite n also multiplies res_b), 3 = C (n = j, composite
-// n only).  Composite flags come from a bitmap over n - nbase.
+// One lane owns K candidates and keeps their residues in VGPRs for a whole
+// launch; the multiplier stream is uniform across the wavefront and lives in
+// SGPRs.  Walk modes: 0 = F (n = j), 1 = P (n = list[j]), 2 = F|C (n = j;
+// composite n also multiplies res_b), 3 = C (n = j, composite n only).
+// Composite flags come from a bitmap over n - nbase, one 32-bit word per 32
+// steps.
+template <int K>
+struct Lanes {
+    uint32_t d1[K], d0[K], v[K], s[K];
+    uint64_t d[K], one[K], nmo[K];
+    bool live[K];
+};
+
+template <int K>
+__device__ __forceinline__ void lanes_from_state(Lanes<K>& L, uint64_t p, uint32_t v, uint64_t one,
+                                                 int k) {
+    L.s[k] = pfc::clz64(p);
+    L.d[k] = p << L.s[k];
+    L.d1[k] = (uint32_t)(L.d[k] >> 32);
+    L.d0[k] = (uint32_t)L.d[k];
+    L.v[k] = v;
+    L.one[k] = one;
+    L.nmo[k] = L.d[k] - one;
+}
+
+// r[k] <- r[k] * n mod d[k] for all k; returns true if any residue is +-1.
+template <int K>
+__device__ __forceinline__ bool step(uint64_t (&r)[K], uint32_t n, const Lanes<K>& L) {
+    bool fix = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        bool f;
+        r[k] = pfc::mulmod32_nofix(r[k], n, L.d1[k], L.d0[k], L.d[k], L.v[k], &f);
+        fix |= f;
+    }
+    // The second correction of the 3-by-2 division is rare: take it as a
+    // wavefront-uniform branch instead of selects on every step.
+    if (__builtin_expect(__ballot(fix) != 0, 0)) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (r[k] >= L.d[k]) r[k] -= L.d[k];
+    }
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) hit |= (r[k] == L.one[k]) | (r[k] == L.nmo[k]);
+    return hit;
+}
+
+template <int K>
+__device__ __forceinline__ void record_hits(const uint64_t (&r)[K], uint32_t n, int32_t type,
+                                            const Lanes<K>& L, pfc_factor* fac, uint32_t fcap,
+                                            pfc_ctrl* ctrl) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (L.live[k] && (r[k] == L.one[k] || r[k] == L.nmo[k]))
+            record_factor(fac, fcap, ctrl, L.d[k] >> L.s[k], n, r[k] == L.nmo[k], type);
+}
+
+struct WalkStream {
+    const uint32_t* list;       // P: multipliers
+    const uint32_t* composite;  // C, F|C: composite bitmap over n - nbase
+    uint32_t nbase;
+    pfc_factor* fac;
+    uint32_t fcap;
+    pfc_ctrl* ctrl;
+};
+
+template <int MODE, int K>
+__device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K], const Lanes<K>& L,
+                                           uint32_t j0, uint32_t j1, const WalkStream& W) {
+    if (MODE == 0 || MODE == 1) {
+        const int32_t type = MODE == 1 ? PFC_PRIMORIAL : PFC_FACTORIAL;
+        const ConstU32 list = (ConstU32)W.list;
+        for (uint32_t j = j0; j < j1; ++j) {
+            const uint32_t n = __builtin_amdgcn_readfirstlane(MODE == 1 ? list[j] : j);
+            if (step<K>(ra, n, L)) record_hits<K>(ra, n, type, L, W.fac, W.fcap, W.ctrl);
+        }
+    } else {
+        const ConstU32 comp = (ConstU32)W.composite;
+        uint32_t j = j0;
+        while (j < j1) {
+            const uint32_t off = j - W.nbase;
+            const uint32_t wbase = off & ~31u;
+            const uint32_t jend = min(j1, W.nbase + wbase + 32u);
+            const uint32_t word = __builtin_amdgcn_readfirstlane(comp[wbase >> 5]);
+            for (; j < jend; ++j) {
+                const uint32_t n = __builtin_amdgcn_readfirstlane(j);
+                const bool composite = (word >> ((n - W.nbase) & 31u)) & 1u;
+                if (MODE == 2) {
+                    if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_FACTORIAL, L, W.fac, W.fcap, W.ctrl);
+                    if (composite && step<K>(rb, n, L))
+                        record_hits<K>(rb, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
+                } else if (composite) {
+                    if (step<K>(ra, n, L))
+                        record_hits<K>(ra, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
+                }
+            }
+        }
+    }
+}
+
+__host__ __device__ constexpr int walk_mode(uint32_t modes) {
+    return modes == PFC_MODE_FACTORIAL ? 0
+         : modes == PFC_MODE_PRIMORIAL ? 1
+         : modes == (PFC_MODE_FACTORIAL | PFC_MODE_COMPOSITORIAL) ? 2
+         : modes == PFC_MODE_COMPOSITORIAL ? 3 : -1;
+}
+
 template <int MODE, int K>
 __global__ __launch_bounds__(256) void k_walk(const WalkArgs A) {
     uint32_t n_c = A.ctrl->ncand;
@@ -233,70 +292,22 @@ __global__ __launch_bounds__(256) void k_walk(const WalkArgs A) {
     const uint32_t first = lane_g * K;
     if (first >= n_c) return;
 
-    Modulus m[K];
-    uint64_t ra[K], rb[K], one[K], nmo[K];
-    bool live[K];
+    Lanes<K> L;
+    uint64_t ra[K], rb[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const uint32_t idx = first + k;
-        live[k] = idx < n_c;
-        const uint32_t src = live[k] ? idx : first;
-        const uint64_t p = A.P[src];
-        m[k].s = pfc::clz64(p);
-        const uint64_t d = p << m[k].s;
-        m[k].d1 = (uint32_t)(d >> 32);
-        m[k].d0 = (uint32_t)d;
-        m[k].v = A.V[src];
-        one[k] = A.ONE[src];
-        nmo[k] = d - one[k];
+        L.live[k] = idx < n_c;
+        const uint32_t src = L.live[k] ? idx : first;
+        lanes_from_state<K>(L, A.P[src], A.V[src], A.ONE[src], k);
         ra[k] = A.RA[src];
         rb[k] = (MODE == 2) ? A.RB[src] : 0;
     }
-    const int32_t type_a = MODE == 1 ? PFC_PRIMORIAL : MODE == 3 ? PFC_COMPOSITORIAL : PFC_FACTORIAL;
-
-    for (uint32_t j = A.j0; j < A.j1; ++j) {
-        // readfirstlane pins the multiplier to one 32-bit SGPR (the loop
-        // counter would otherwise be widened to 64 bits).
-        const uint32_t n = __builtin_amdgcn_readfirstlane((MODE == 1) ? A.list[j] : j);
-        if (MODE == 3) {
-            const uint32_t off = n - A.nbase;
-            if (!((A.composite[off >> 5] >> (off & 31)) & 1u)) continue;
-        }
-        bool hit = false;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            ra[k] = pfc::mulmod32(ra[k], n, m[k]);
-            hit |= (ra[k] == one[k]) | (ra[k] == nmo[k]);
-        }
-        if (hit) {
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (live[k] && (ra[k] == one[k] || ra[k] == nmo[k]))
-                    record_factor(A.fac, A.fcap, A.ctrl, pfc::mod_d(m[k]) >> m[k].s, n,
-                                  ra[k] == nmo[k], type_a);
-        }
-        if (MODE == 2) {
-            const uint32_t off = n - A.nbase;
-            if ((A.composite[off >> 5] >> (off & 31)) & 1u) {
-                bool hitc = false;
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    rb[k] = pfc::mulmod32(rb[k], n, m[k]);
-                    hitc |= (rb[k] == one[k]) | (rb[k] == nmo[k]);
-                }
-                if (hitc) {
-#pragma unroll
-                    for (int k = 0; k < K; ++k)
-                        if (live[k] && (rb[k] == one[k] || rb[k] == nmo[k]))
-                            record_factor(A.fac, A.fcap, A.ctrl, pfc::mod_d(m[k]) >> m[k].s, n,
-                                          rb[k] == nmo[k], PFC_COMPOSITORIAL);
-                }
-            }
-        }
-    }
+    const WalkStream W{A.list, A.composite, A.nbase, A.fac, A.fcap, A.ctrl};
+    walk_steps<MODE, K>(ra, rb, L, A.j0, A.j1, W);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        if (!live[k]) continue;
+        if (!L.live[k]) continue;
         A.RA[first + k] = ra[k];
         if (MODE == 2) A.RB[first + k] = rb[k];
     }
@@ -305,8 +316,24 @@ __global__ __launch_bounds__(256) void k_walk(const WalkArgs A) {
 // ------------------------------------------------------------------ check
 
 // Term per candidate (check.cl:59,115,161): F or C: Mont(X) + Mont(nmax-1);
-// F|C: Mont(F) + Mont(C) + Mont(nmax-1); P: Mont(X).  Block sums are added to
-// ctrl->checksum with one 64-bit atomic per block (wrapping, order free).
+// F|C: Mont(F) + Mont(C) + Mont(nmax-1); P: Mont(X).
+__device__ __forceinline__ uint64_t check_term(uint32_t modes, uint32_t nmax, const Modulus& m,
+                                               uint64_t one, uint64_t ra, uint64_t rb) {
+    const uint64_t a = ra >> m.s;
+    if (modes & PFC_MODE_PRIMORIAL) return a;
+    uint64_t t = a + (pfc::mulmod32(one, nmax - 1, m) >> m.s);
+    if ((modes & PFC_MODE_FACTORIAL) && (modes & PFC_MODE_COMPOSITORIAL)) t += rb >> m.s;
+    return t;
+}
+
+// Wavefront sum of v, added to *dst by lane 0 of each wavefront (wrapping).
+__device__ __forceinline__ void wave_add_u64(uint64_t v, uint64_t* dst) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (__lane_id() == 0) atomicAdd((unsigned long long*)dst, (unsigned long long)v);
+}
+
+// Block sums are added to ctrl->checksum with one 64-bit atomic per
+// wavefront; the validation flag checks every stored reciprocal.
 __global__ __launch_bounds__(256) void k_check(uint32_t modes, uint32_t nmax,
                                                const uint64_t* __restrict__ P,
                                                const uint32_t* __restrict__ V,
@@ -314,8 +341,7 @@ __global__ __launch_bounds__(256) void k_check(uint32_t modes, uint32_t nmax,
                                                const uint64_t* __restrict__ RA,
                                                const uint64_t* __restrict__ RB, pfc_ctrl* ctrl,
                                                uint32_t cap) {
-    __shared__ uint64_t s_part[256 / 64];
-    uint32_t n = ctrl->ncand;
+    const uint32_t n = ctrl->ncand;
     const uint32_t ncap = n > cap ? cap : n;
     uint64_t sum = 0;
     bool bad = false;
@@ -328,25 +354,97 @@ __global__ __launch_bounds__(256) void k_check(uint32_t modes, uint32_t nmax,
         m.d0 = (uint32_t)d;
         m.v = V[i];
         bad |= !pfc::modulus_ok(p, m);
-        const uint64_t a = RA[i] >> m.s;
-        if (modes & PFC_MODE_PRIMORIAL) {
-            sum += a;
-        } else {
-            const uint64_t last = pfc::mulmod32(ONE[i], nmax - 1, m) >> m.s;
-            sum += a + last;
-            if ((modes & PFC_MODE_FACTORIAL) && (modes & PFC_MODE_COMPOSITORIAL)) sum += RB[i] >> m.s;
-        }
+        sum += check_term(modes, nmax, m, ONE[i], RA[i], RB ? RB[i] : 0);
     }
     if (bad) atomicOr(&ctrl->flags, 2u);
-    // Wavefront sum, then one atomic per block.
-    for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off);
-    if (__lane_id() == 0) s_part[threadIdx.x >> 6] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t t = 0;
-        for (uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w) t += s_part[w];
-        atomicAdd((unsigned long long*)&ctrl->checksum, (unsigned long long)t);
-        if (blockIdx.x == 0) atomicAdd((unsigned long long*)&ctrl->primecount, (unsigned long long)n);
+    wave_add_u64(sum, &ctrl->checksum);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd((unsigned long long*)&ctrl->primecount, (unsigned long long)n);
+}
+
+// ------------------------------------------------------------------ batch
+
+// Persistent fused kernel over a device work queue.  One workgroup is one
+// wavefront; it takes items (job, 64*K candidates, step range) from the
+// queue until the queue is drained.  An item with kBatchSetup first builds
+// the candidates' reciprocals and start residues (as k_setup); an item with
+// kBatchCheck finally folds their checksum terms (as k_check).  Items of one
+// launch may come from different jobs with different modes, tables and
+// n-ranges: each wavefront's stream is uniform, the launch is not.
+constexpr uint32_t kBatchSetup = 1u, kBatchCheck = 2u;
+
+template <int K>
+__device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_batch_item& it) {
+    uint32_t n_c = J.ctrl->ncand;
+    if (n_c > J.cap) n_c = J.cap;
+    const uint32_t first = it.first + __lane_id() * K;
+    Lanes<K> L;
+    uint64_t ra[K], rb[K];
+    const bool two = J.modes == (PFC_MODE_FACTORIAL | PFC_MODE_COMPOSITORIAL);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t idx = first + k;
+        L.live[k] = idx < n_c;
+        const uint32_t src = L.live[k] ? idx : it.first;   // items never start past ncand
+        const uint64_t p = J.P[src];
+        if (it.flags & kBatchSetup) {
+            const Modulus m = pfc::make_modulus(p);
+            const uint64_t one = pfc::mont_one(m);
+            lanes_from_state<K>(L, p, m.v, one, k);
+            ra[k] = horner_start(J.ca, J.ga, J.nba, m, one);
+            rb[k] = two ? horner_start(J.cb, J.gb, J.nbb, m, one) : 0;
+        } else {
+            lanes_from_state<K>(L, p, J.V[src], J.ONE[src], k);
+            ra[k] = J.RA[src];
+            rb[k] = two ? J.RB[src] : 0;
+        }
+    }
+    const WalkStream W{J.list, J.composite, J.nmin, J.fac, J.fcap, J.ctrl};
+    switch (walk_mode(J.modes)) {
+        case 0: walk_steps<0, K>(ra, rb, L, it.j0, it.j1, W); break;
+        case 1: walk_steps<1, K>(ra, rb, L, it.j0, it.j1, W); break;
+        case 2: walk_steps<2, K>(ra, rb, L, it.j0, it.j1, W); break;
+        default: walk_steps<3, K>(ra, rb, L, it.j0, it.j1, W); break;
+    }
+    uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (!L.live[k]) continue;
+        const uint32_t idx = first + k;
+        J.RA[idx] = ra[k];
+        if (two) J.RB[idx] = rb[k];
+        if (it.flags & kBatchSetup) {
+            J.V[idx] = L.v[k];
+            J.ONE[idx] = L.one[k];
+        }
+        if (it.flags & kBatchCheck) {
+            Modulus m;
+            m.d1 = L.d1[k];
+            m.d0 = L.d0[k];
+            m.v = L.v[k];
+            m.s = L.s[k];
+            sum += check_term(J.modes, J.nmax, m, L.one[k], ra[k], rb[k]);
+        }
+    }
+    if (it.flags & kBatchCheck) {
+        wave_add_u64(sum, &J.ctrl->checksum);
+        if (it.first == 0 && __lane_id() == 0)
+            atomicAdd((unsigned long long*)&J.ctrl->primecount, (unsigned long long)n_c);
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void k_batch(const pfc_batch_job* __restrict__ jobs,
+                                              const pfc_batch_item* __restrict__ items,
+                                              uint32_t nitems, uint32_t* head) {
+    for (;;) {
+        uint32_t i = 0;
+        if (__lane_id() == 0) i = atomicAdd(head, 1u);
+        i = __builtin_amdgcn_readfirstlane(__shfl(i, 0));
+        if (i >= nitems) break;
+        const pfc_batch_item it = items[i];
+        const pfc_batch_job J = jobs[it.job];
+        batch_item<K>(J, it);
     }
 }
 
@@ -488,6 +586,18 @@ int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint3
                        d_rb, d_ctrl, cap);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_check launch");
+}
+
+int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_item* d_items, uint32_t nitems,
+                 uint32_t* d_head, uint32_t ppl, uint32_t nblocks, hipStream_t st) {
+    if (nitems == 0) return PFC_OK;
+    if (nblocks == 0) nblocks = 1;
+    if (ppl == 1) hipLaunchKernelGGL(k_batch<1>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_items, nitems, d_head);
+    else if (ppl == 2) hipLaunchKernelGGL(k_batch<2>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_items, nitems, d_head);
+    else if (ppl == 4) hipLaunchKernelGGL(k_batch<4>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_items, nitems, d_head);
+    else return set_error(PFC_ERR_INVALID, "batch: primes_per_lane must be 1, 2 or 4");
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "k_batch launch");
 }
 
 int probe_mul_rates(int cus, double rates[3]) {

@@ -47,6 +47,12 @@ class Device:
         check(lib().pfc_get_tuning(self._h, C.byref(t)))
         return {f: getattr(t, f) for f, _ in Tuning._fields_}
 
+    def autotune(self, modes, pmin, pmax, nmin, nmax) -> dict:
+        d = job_desc(modes, pmin, pmax, nmin, nmax)
+        t = Tuning()
+        check(lib().pfc_autotune(self._h, C.byref(d), C.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}
+
     def probe_mul_rates(self):
         r = (C.c_double * 3)()
         check(lib().pfc_probe_mul_rates(self._h, C.byref(r)))
@@ -175,3 +181,71 @@ def run_sieve(modes, pmin, pmax, nmin, nmax, *, device: Optional[Device] = None,
         return res
     finally:
         job.destroy()
+
+
+class Batcher:
+    """Continuous batching of jobs on one device (pfc_batcher_* in include/pfc.h)."""
+
+    def __init__(self, dev: Device):
+        self.dev = dev
+        h = C.c_void_p()
+        check(lib().pfc_batcher_create(dev.handle, C.byref(h)))
+        self._h = h
+        self._jobs = {}
+
+    def add(self, job: Job):
+        check(lib().pfc_batcher_add(self._h, job.handle))
+        self._jobs[job.handle.value] = job
+
+    def pending(self) -> int:
+        n = C.c_size_t()
+        check(lib().pfc_batcher_pending(self._h, C.byref(n)))
+        return n.value
+
+    def round(self, max_steps: int = 0) -> List[Job]:
+        cap = max(len(self._jobs), 1)
+        arr = (C.c_void_p * cap)()
+        nd = C.c_size_t()
+        check(lib().pfc_batcher_round(self._h, max_steps, arr, cap, C.byref(nd)))
+        return [self._jobs.pop(arr[i]) for i in range(min(nd.value, cap))]
+
+    def destroy(self):
+        if self._h:
+            lib().pfc_batcher_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def job_result(job: Job, verify_threads: int = 2) -> SieveResult:
+    """Harvested result of a finished job (the tail of run_sieve)."""
+    st = job.stats()
+    arr, n = job.raw_factors()
+    checksum, kept = harvest(arr, n, st.dev_checksum, verify_threads)
+    res = SieveResult(factorcount=len(kept), primecount=st.primecount, checksum=checksum,
+                      device_ms=st.device_ms, segments=st.segments)
+    res.lines = [format_factor(f) for f in kept]
+    res.factors = [(f.p, f.nc, f.type) for f in kept]
+    return res
+
+
+def run_batch(descs, *, device: Optional[Device] = None, max_steps: int = 0,
+              verify_threads: int = 2) -> List[SieveResult]:
+    """Runs job descriptions (modes, pmin, pmax, nmin, nmax) through one batcher."""
+    dev = device or Device(0)
+    jobs = [Job(dev, *d) for d in descs]
+    b = Batcher(dev)
+    try:
+        for j in jobs:
+            b.add(j)
+        while b.pending():
+            b.round(max_steps)
+        return [job_result(j, verify_threads) for j in jobs]
+    finally:
+        b.destroy()
+        for j in jobs:
+            j.destroy()

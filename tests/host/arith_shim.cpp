@@ -1,0 +1,38 @@
+// Host build of the device arithmetic (pfcsieve_amd/csrc/pfc_arith.h) so the
+// CPU tests can check it against Python big integers.  Test-only.
+#include <stddef.h>
+#include <stdint.h>
+#include "../../pfcsieve_amd/csrc/pfc_arith.h"
+
+extern "C" {
+void t_modulus(uint64_t p, uint32_t out[4]) {
+    pfc::Modulus m = pfc::make_modulus(p);
+    out[0] = m.d1; out[1] = m.d0; out[2] = m.v; out[3] = m.s;
+}
+uint64_t t_mulmod32(uint64_t p, uint64_t a_shifted, uint32_t n) {
+    return pfc::mulmod32(a_shifted, n, pfc::make_modulus(p));
+}
+uint64_t t_mulmod64(uint64_t p, uint64_t a_shifted, uint64_t b) {
+    return pfc::mulmod64(a_shifted, b, pfc::make_modulus(p));
+}
+uint64_t t_sqrmod(uint64_t p, uint64_t a_shifted) { return pfc::sqrmod(a_shifted, pfc::make_modulus(p)); }
+uint64_t t_mont_one(uint64_t p) { return pfc::mont_one(pfc::make_modulus(p)); }
+int t_modulus_ok(uint64_t p) { return pfc::modulus_ok(p, pfc::make_modulus(p)); }
+int t_strong_prp2(uint64_t n) { return pfc::strong_prp2(n); }
+// Candidates of [lo, hi) (lo >= 114) exactly as k_generate's lanes compute them.
+size_t t_generate(uint64_t lo, uint64_t hi, uint64_t* out, size_t cap) {
+    size_t n = 0;
+    const uint64_t base30 = lo - lo % 30;
+    for (uint64_t start = base30; start < hi && start >= base30; start += 60) {
+        uint32_t rmod[27];
+        for (int i = 0; i < 27; ++i) rmod[i] = (uint32_t)(start % pfc::kSievePrimes[i]);
+        uint64_t c = pfc::window_survivors(start, lo, hi, rmod);
+        while (c) {
+            const int j = __builtin_ctzll(c);
+            c &= c - 1;
+            if (pfc::strong_prp2(start + j)) { if (n < cap) out[n] = start + j; ++n; }
+        }
+    }
+    return n;
+}
+}

@@ -1,0 +1,69 @@
+"""Batched and continuously batched execution: every job's result equals the
+oracle's (and therefore running it alone), whatever it shares launches with."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+F, P, Cm = 1, 2, 4
+R = 1 << 64
+
+MIX = [
+    (F, 10**9, 10**9 + 3 * 10**5, 101, 3000),
+    (P, 10**12, 10**12 + 2 * 10**5, 2000, 9000),
+    (Cm, 101, 40000, 101, 2500),
+    (F | Cm, (1 << 62) - 10**5, 1 << 62, 500, 1800),
+    (F, R - 10**5, R - 1, 101, 700),
+    (P, 101, 20000, 101, 4000),
+    (Cm, 10**7, 10**7 + 10**5, 5000, 7000),
+    (F, 3, 500, 101, 400),
+]
+
+
+def _ref(orc, job):
+    r = orc.run(*job)
+    return r.checksum, r.primecount, [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in r.factors]
+
+
+@pytest.mark.parametrize("max_steps", [0, 700])
+def test_batch_matches_oracle(orc, dev, max_steps):
+    import pfcsieve_amd as PK
+    got = PK.run_batch(MIX, device=dev, max_steps=max_steps, verify_threads=8)
+    for job, g in zip(MIX, got):
+        cs, pc, fac = _ref(orc, job)
+        assert (g.checksum, g.primecount, g.factors) == (cs, pc, fac), job
+
+
+def test_batch_multi_segment_jobs(orc, dev):
+    import pfcsieve_amd as PK
+    dev.set_tuning(seg_candidates=3000)
+    try:
+        got = PK.run_batch(MIX[:4], device=dev, max_steps=1500, verify_threads=8)
+    finally:
+        dev.set_tuning()
+    for job, g in zip(MIX[:4], got):
+        cs, pc, fac = _ref(orc, job)
+        assert (g.checksum, g.primecount, g.factors) == (cs, pc, fac), job
+        assert g.segments > 1
+
+
+def test_continuous_admission(orc, dev):
+    """Jobs added while others are mid-walk finish with their standalone result."""
+    import pfcsieve_amd as PK
+    b = PK.Batcher(dev)
+    jobs = [PK.Job(dev, *d) for d in MIX]
+    done = []
+    try:
+        for k, j in enumerate(jobs):
+            b.add(j)
+            done += b.round(max_steps=500)      # admit one job per round
+        while b.pending():
+            done += b.round(max_steps=500)
+        assert len(done) == len(jobs)
+        for d, j in zip(MIX, jobs):
+            g = PK.job_result(j, verify_threads=8)
+            cs, pc, fac = _ref(orc, d)
+            assert (g.checksum, g.primecount, g.factors) == (cs, pc, fac), d
+    finally:
+        b.destroy()
+        for j in jobs:
+            j.destroy()

@@ -1,0 +1,211 @@
+"""CPU-side checks of libpfcsieve.so: the exported C ABI, its host functions
+(isPrime, number grammar, harvest, factor lines, setup tables) and the device
+arithmetic compiled for the host.  No GPU calls."""
+import ctypes as C
+import math
+import os
+import random
+import re
+import subprocess
+
+import numpy as np
+import pytest
+import sympy
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+R = 1 << 64
+
+
+@pytest.fixture(scope="module")
+def L():
+    from pfcsieve_amd._native import lib
+    return lib()
+
+
+def test_every_header_symbol_exported(L):
+    hdr = open(os.path.join(ROOT, "include", "pfc.h")).read()
+    names = set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(pfc_\w+)\(", hdr, re.M))
+    assert len(names) >= 25
+    from pfcsieve_amd._native import SIGNATURES
+    assert names == set(SIGNATURES), names ^ set(SIGNATURES)
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_is_prime(L):
+    rng = random.Random(1)
+    xs = list(range(0, 3000)) + [rng.randrange(3, R) for _ in range(3000)]
+    xs += [3215031751, 3825123056546413051, 18446744073709551557, 2**61 - 1, 2**64 - 1]
+    for x in xs:
+        assert bool(L.pfc_is_prime(x)) == sympy.isprime(x), x
+
+
+GRAMMAR = ["101", "1e6", "1E6", "25M", "2G", "3T", "1P", "5K", "1k", "1m", "2g", "1t", "1p", "1b10",
+           "3B4", "0x10", "010", "12abc", "1e", "1ee", "1KK", "18446744073709551615", "18446744073709551616",
+           "2e19", "1e20", "17b60", "1b64", "", "-5", "4611686018427387904", "100e7", "1730720716e6"]
+
+
+def test_parse_grammar_against_reference(L, orc):
+    for s in GRAMMAR:
+        for lo, hi in [(3, R - 2), (4, R - 1), (101, 0x7FFFFFFE), (1, 128)]:
+            out = C.c_uint64()
+            rc = L.pfc_parse_uint64(s.encode(), lo, hi, C.byref(out))
+            if orc.ref_available():
+                rrc, rv = orc.ref_parse_uint64(s, lo, hi)
+                assert rc == rrc, (s, lo, hi)
+                if rc == 0:
+                    assert out.value == rv, s
+
+
+def test_parse_grammar_values(L):
+    def p(s, lo=0, hi=R - 1):
+        out = C.c_uint64()
+        rc = L.pfc_parse_uint64(s.encode(), lo, hi, C.byref(out))
+        return out.value if rc == 0 else rc
+    assert p("1e6") == 10**6 and p("25M") == 25 * 10**6 and p("3T") == 3 * 10**12
+    assert p("1k") == 1024 and p("1b10") == 1024 and p("0x10") == 16 and p("010") == 8
+    assert p("12abc") == -1 and p("1e20") == -2 and p("50", lo=101) == -2
+
+
+def test_format_factor(L):
+    from pfcsieve_amd._native import Factor
+    buf = C.create_string_buffer(96)
+    for f, line in [((1000003, -200, 0), "1000003 | 200!-1\n"), ((17, 5, 1), "17 | 5#+1\n"),
+                    ((2**64 - 59, 300, 2), "18446744073709551557 | 300!/#+1\n")]:
+        L.pfc_format_factor(C.byref(Factor(*f)), buf, 96)
+        assert buf.value.decode() == line
+
+
+def test_harvest_matches_oracle(L, orc):
+    from pfcsieve_amd._native import Factor
+    # raw records of a job with pseudoprime-free small p, shuffled
+    r = orc.run(1 | 4, 101, 5000, 101, 3000)
+    raw = [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in r.factors]
+    # add records of a base-2 pseudoprime (composite p) that must be dropped
+    raw += [(42799, 1234, 0), (42799, -2000, 2)]
+    random.Random(2).shuffle(raw)
+    arr = (Factor * len(raw))(*[Factor(*x) for x in raw])
+    cs = C.c_uint64(r.dev_checksum)
+    kept = C.c_size_t()
+    assert L.pfc_harvest(arr, len(raw), 0, C.byref(cs), C.byref(kept)) == 0
+    assert cs.value == r.checksum
+    assert [(arr[i].p, arr[i].nc, arr[i].type) for i in range(kept.value)] == \
+        [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in r.factors]
+
+
+def test_harvest_cpu_verify_rejects_bad_record(L):
+    from pfcsieve_amd._native import Factor
+    good = [Factor(103, 102, 0), Factor(103, -101, 0)]          # Wilson: 102!+1, 101!-1
+    arr = (Factor * 2)(*good)
+    cs, kept = C.c_uint64(0), C.c_size_t()
+    assert L.pfc_harvest(arr, 2, 2, C.byref(cs), C.byref(kept)) == 0 and kept.value == 2
+    bad = (Factor * 1)(Factor(103, 101, 0))
+    assert L.pfc_harvest(bad, 1, 2, C.byref(cs), C.byref(kept)) == -6
+    assert b"is not a factor of 101!+1" in L.pfc_last_error()
+
+
+def test_verify_factor(L):
+    assert L.pfc_verify_factor(103, 102, 1, 0) and L.pfc_verify_factor(103, 101, -1, 0)
+    assert not L.pfc_verify_factor(103, 101, 1, 0)
+    for n in range(101, 400):
+        for p in [1000003, 10007]:
+            for t, X in [(0, math.factorial(n)), (1, sympy.primorial(n, nth=False)),
+                         (2, math.factorial(n) // sympy.primorial(n, nth=False))]:
+                r = X % p
+                assert bool(L.pfc_verify_factor(p, n, -1, t)) == (r == 1)
+                assert bool(L.pfc_verify_factor(p, n, 1, t)) == (r == p - 1)
+
+
+@pytest.mark.parametrize("kind,N", [(0, 100), (0, 4999), (0, 100000), (1, 100), (1, 99999),
+                                    (2, 100), (2, 77777)])
+def test_setup_table_reconstructs_start_value(L, kind, N):
+    nch, nb = C.c_size_t(), C.c_uint32()
+    L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb))
+    ch = (C.c_uint32 * nch.value)()
+    gr = (C.c_uint32 * (nb.value + 1))()
+    assert L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb)) == 0
+    x = 1
+    for b in range(nb.value - 1, -1, -1):      # Horner over exponent bits, high first
+        x = x * x
+        for k in range(gr[b], gr[b + 1]):
+            x *= ch[k]
+    prim = sympy.primorial(N, nth=False)
+    expect = [math.factorial(N), prim, math.factorial(N) // prim][kind]
+    assert x == expect
+    assert all(c < 2**32 for c in ch)
+
+
+# ----------------------------------------------------------------- device arithmetic on the host
+
+
+@pytest.fixture(scope="module")
+def A():
+    out = os.path.join(ROOT, "tests", "host", "_build", "libarith.so")
+    src = os.path.join(ROOT, "tests", "host", "arith_shim.cpp")
+    hdr = os.path.join(ROOT, "pfcsieve_amd", "csrc", "pfc_arith.h")
+    if not os.path.exists(out) or os.path.getmtime(out) < max(os.path.getmtime(src), os.path.getmtime(hdr)):
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        subprocess.run(["g++", "-O2", "-fPIC", "-shared", "-o", out, src], check=True)
+    L = C.CDLL(out)
+    u64, u32 = C.c_uint64, C.c_uint32
+    L.t_modulus.argtypes = [u64, C.POINTER(u32 * 4)]
+    for n, args in [("t_mulmod32", [u64, u64, u32]), ("t_mulmod64", [u64, u64, u64]),
+                    ("t_sqrmod", [u64, u64]), ("t_mont_one", [u64])]:
+        getattr(L, n).argtypes = args
+        getattr(L, n).restype = u64
+    L.t_modulus_ok.argtypes = [u64]
+    L.t_strong_prp2.argtypes = [u64]
+    L.t_generate.argtypes = [u64, u64, C.POINTER(u64), C.c_size_t]
+    L.t_generate.restype = C.c_size_t
+    return L
+
+
+def _primes_mix(rng, k):
+    out = [3, 5, 7, 113, 65537, 2**32 - 5, 2**32 + 15, 2**61 - 1, 2**62 - 57, 2**63 - 25, 2**63 + 29,
+           2**64 - 59, 1000003, 10**12 + 39]
+    out += [rng.randrange(3, R) | 1 for _ in range(k)]
+    return [p if p < R else p - 2 for p in out]
+
+
+def test_modulus_and_reciprocal(A):
+    rng = random.Random(3)
+    for p in _primes_mix(rng, 2000):
+        m = (C.c_uint32 * 4)()
+        A.t_modulus(p, C.byref(m))
+        s = 64 - p.bit_length()
+        d = p << s
+        assert m[3] == s and (m[0] << 32 | m[1]) == d
+        assert m[2] == ((1 << 96) - 1) // d - (1 << 32)
+        assert A.t_modulus_ok(p)
+        assert A.t_mont_one(p) == (R % p) << s
+
+
+def test_mulmod_against_bigint(A):
+    rng = random.Random(4)
+    for p in _primes_mix(rng, 3000):
+        s = 64 - p.bit_length()
+        for a in [0, 1, p - 1, rng.randrange(p), rng.randrange(p)]:
+            for n in [1, 2, 0x7FFFFFFF, 0xFFFFFFFF, rng.randrange(1 << 32)]:
+                assert A.t_mulmod32(p, a << s, n) == (a * n % p) << s
+            b = rng.randrange(R)
+            assert A.t_mulmod64(p, a << s, b) == (a * b % p) << s
+            assert A.t_sqrmod(p, a << s) == (a * a % p) << s
+
+
+def test_strong_prp2_device_arith(A, orc):
+    rng = random.Random(5)
+    xs = list(range(3, 3000, 2)) + [rng.randrange(3, R) | 1 for _ in range(3000)]
+    xs += [(1 << t) + 1 for t in (8, 16, 32, 40, 48, 56)] + [3215031751, 2152302898747, 3474749660383]
+    for x in xs:
+        assert bool(A.t_strong_prp2(x)) == orc.strong_prp2(x), x
+
+
+@pytest.mark.parametrize("lo,hi", [(114, 20000), (119, 120), (10**9, 10**9 + 2 * 10**5),
+                                   (2**62 - 10**5, 2**62), (R - 10**5, R - 1)])
+def test_generate_window_logic_matches_oracle(A, orc, lo, hi):
+    """The generator's window arithmetic (sieve masks, range clipping) as the
+    kernel lanes run it, against the oracle's candidate set."""
+    n = A.t_generate(lo, hi, None, 0)
+    out = (C.c_uint64 * max(n, 1))()
+    A.t_generate(lo, hi, out, n)
+    assert list(out[:n]) == [int(x) for x in orc.candidates(lo, hi)]
