@@ -72,6 +72,8 @@ typedef struct pfc_tuning {
     uint32_t block_size;       /* walk workgroup size: 64, 128, 256 */
     uint32_t walk_chunk;       /* multiplier steps per walk launch */
     uint32_t seg_candidates;   /* target candidates per p-segment */
+    uint32_t use_graph;        /* 1: capture a job's segment pipeline (clear, generate, setup,
+                                  walk chunks, check) once as a HIP graph and replay it per segment */
 } pfc_tuning;
 
 /* Host-side summary of a finished job (cl_sieve.cpp:1748). */
@@ -81,6 +83,9 @@ typedef struct pfc_job_stats {
     uint64_t raw_factors;      /* factor records before the isPrime filter */
     uint64_t segments;         /* p-segments processed */
     double device_ms;          /* stream time of the whole job */
+    double walk_ms;            /* summed duration of the walk launches (HIP events) */
+    uint64_t walk_launches;    /* walk kernel launches */
+    uint64_t walk_updates;     /* p*n residue updates done by those launches */
 } pfc_job_stats;
 
 typedef struct pfc_device pfc_device;
@@ -186,6 +191,15 @@ typedef struct pfc_ctrl {
     uint64_t primecount; /* sum of ncand over checked segments */
 } pfc_ctrl;
 
+/* Device memory for stage-level callers that have no allocator of their own
+ * (plain hipMalloc/hipFree/hipMemcpy on the device's ordinal).  kind: 1 host
+ * to device, 2 device to host, 3 device to device.  pfc_device_synchronize
+ * waits for all work on the device. */
+int pfc_device_malloc(pfc_device* dev, size_t bytes, void** ptr);
+int pfc_device_free(pfc_device* dev, void* ptr);
+int pfc_memcpy(pfc_device* dev, void* dst, const void* src, size_t bytes, int kind);
+int pfc_device_synchronize(pfc_device* dev);
+
 /* (a) Candidates in [lo, hi): primes below 114 plus wheel-30 numbers with no
  * factor 7..113 that pass the base-2 strong PRP (getsegprimes.cl:464-526,
  * addsmallprimes.cl:21-40).  Appends to d_p[0..cap) in no particular order;
@@ -194,9 +208,9 @@ int pfc_stage_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pf
                        void* stream);
 
 /* Host-built start-residue tables (replacing cl_sieve.cpp:842-1326): chunks
- * are products of primes below 2^32, grouped by exponent bit, high bit first. */
+ * are products of primes below 2^64, grouped by exponent bit, high bit first. */
 typedef struct pfc_setup_table {
-    const uint32_t* chunks;    /* device */
+    const uint64_t* chunks;    /* device */
     const uint32_t* groups;    /* device: nbits+1 offsets into chunks, group b = bit b */
     uint32_t nbits;
 } pfc_setup_table;
@@ -229,7 +243,7 @@ int pfc_stage_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const ui
 
 /* Builds the setup table for X(N) on the host: kind 0 = N!, 1 = N#, 2 = N!/N#.
  * Writes up to chunk_cap chunks and nbits+1 group offsets (group_cap >= 33). */
-int pfc_build_setup_table(uint32_t kind, uint32_t N, uint32_t* chunks, size_t chunk_cap,
+int pfc_build_setup_table(uint32_t kind, uint32_t N, uint64_t* chunks, size_t chunk_cap,
                           size_t* nchunks, uint32_t* groups, size_t group_cap, uint32_t* nbits);
 
 /* ------------------------------------------------------------------ probe */
@@ -237,6 +251,9 @@ int pfc_build_setup_table(uint32_t kind, uint32_t N, uint32_t* chunks, size_t ch
  * v_mul_hi_u32 streams on all CUs (the INT32 multiply roofline).  rates[3]
  * receives lane-ops per second for {mad_u64_u32, mul_lo_u32, mul_hi_u32}. */
 int pfc_probe_mul_rates(pfc_device* dev, double rates[3]);
+/* One launch of a plain device copy of `bytes` (read + write), the reference
+ * kernel for scaling HBM traffic counters; *ms receives its duration. */
+int pfc_probe_copy(pfc_device* dev, uint64_t bytes, double* ms);
 
 #ifdef __cplusplus
 }

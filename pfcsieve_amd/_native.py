@@ -36,12 +36,13 @@ class JobDesc(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("primes_per_lane", C.c_uint32), ("block_size", C.c_uint32),
-                ("walk_chunk", C.c_uint32), ("seg_candidates", C.c_uint32)]
+                ("walk_chunk", C.c_uint32), ("seg_candidates", C.c_uint32), ("use_graph", C.c_uint32)]
 
 
 class JobStats(C.Structure):
     _fields_ = [("dev_checksum", C.c_uint64), ("primecount", C.c_uint64),
-                ("raw_factors", C.c_uint64), ("segments", C.c_uint64), ("device_ms", C.c_double)]
+                ("raw_factors", C.c_uint64), ("segments", C.c_uint64), ("device_ms", C.c_double),
+                ("walk_ms", C.c_double), ("walk_launches", C.c_uint64), ("walk_updates", C.c_uint64)]
 
 
 class Ctrl(C.Structure):
@@ -83,15 +84,20 @@ SIGNATURES = {
     "pfc_is_prime": (_int, [_u64]),
     "pfc_verify_factor": (_int, [_u64, _u32, _i32, _i32]),
     "pfc_parse_uint64": (_int, [C.c_char_p, _u64, _u64, C.POINTER(_u64)]),
+    "pfc_device_malloc": (_int, [_vp, _sz, C.POINTER(_vp)]),
+    "pfc_device_free": (_int, [_vp, _vp]),
+    "pfc_memcpy": (_int, [_vp, _vp, _vp, _sz, _int]),
+    "pfc_device_synchronize": (_int, [_vp]),
     "pfc_stage_generate": (_int, [_u64, _u64, _vp, _u32, _vp, _vp]),
     "pfc_stage_setup": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(SetupTable),
                                C.POINTER(SetupTable), _u32, _vp]),
     "pfc_stage_walk": (_int, [_u32, _u32, _u32, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                               _u32, _u32, _vp]),
     "pfc_stage_check": (_int, [_u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
-    "pfc_build_setup_table": (_int, [_u32, _u32, C.POINTER(_u32), _sz, C.POINTER(_sz),
+    "pfc_build_setup_table": (_int, [_u32, _u32, C.POINTER(_u64), _sz, C.POINTER(_sz),
                                      C.POINTER(_u32), _sz, C.POINTER(_u32)]),
     "pfc_probe_mul_rates": (_int, [_vp, C.POINTER(C.c_double * 3)]),
+    "pfc_probe_copy": (_int, [_vp, _u64, C.POINTER(C.c_double)]),
 }
 
 _lib = None

@@ -108,10 +108,17 @@ PFC_HD uint64_t addmod(uint64_t a, uint64_t b, const Modulus& m) {
     return (r < a || r >= d) ? r - d : r;
 }
 
-// (a * b) mod d for a < d and any 64-bit b (b is a plain integer).
+// (a * b) mod d for a < d and any 64-bit b (b is a plain integer): the
+// 128-bit product <t3,t2,t1,t0>, then two 3-by-2 steps, <t3,t2,t1> mod d and
+// <r,t0> mod d (both satisfy the <u2,u1> < d precondition because a < d).
 PFC_HD uint64_t mulmod64(uint64_t a, uint64_t b, const Modulus& m) {
-    const uint64_t hi = shl32mod(mulmod32(a, (uint32_t)(b >> 32), m), m);
-    return addmod(hi, mulmod32(a, (uint32_t)b, m), m);
+    const uint64_t a0 = (uint32_t)a, a1 = a >> 32, b0 = (uint32_t)b, b1 = b >> 32;
+    const uint64_t p00 = a0 * b0;
+    const uint64_t p01 = a0 * b1 + (p00 >> 32);
+    const uint64_t p10 = a1 * b0 + (uint32_t)p01;
+    const uint64_t p11 = a1 * b1 + (p01 >> 32) + (p10 >> 32);
+    const uint64_t r = div3by2_rem((uint32_t)(p11 >> 32), (uint32_t)p11, (uint32_t)p10, m);
+    return div3by2_rem((uint32_t)(r >> 32), (uint32_t)r, (uint32_t)p00, m);
 }
 
 // Square of the residue a' = x << s: (x*x mod p) << s.

@@ -70,9 +70,9 @@ std::vector<uint32_t> primes_in(uint64_t lo, uint64_t hi) {
 
 // Setup table (include/pfc.h pfc_setup_table): exponents of every prime r <= N
 // in X(N) = N! (kind 0), N# (kind 1) or N!/N# (kind 2); group b holds the
-// primes with bit b of the exponent set, multiplied into chunks below 2^32.
+// primes with bit b of the exponent set, multiplied into chunks below 2^64.
 struct HostTable {
-    std::vector<uint32_t> chunks;
+    std::vector<uint64_t> chunks;
     std::vector<uint32_t> groups;  // nbits + 1 offsets
     uint32_t nbits = 0;
 };
@@ -99,13 +99,13 @@ HostTable build_table(uint32_t kind, uint32_t N) {
         uint64_t acc = 1;
         for (size_t i = 0; i < pr.size(); ++i) {
             if (!((ex[i] >> b) & 1u)) continue;
-            if (acc * pr[i] > 0xFFFFFFFFull) {
-                T.chunks.push_back((uint32_t)acc);
+            if ((u128)acc * pr[i] > 0xFFFFFFFFFFFFFFFFull) {
+                T.chunks.push_back(acc);
                 acc = 1;
             }
             acc *= pr[i];
         }
-        if (acc > 1) T.chunks.push_back((uint32_t)acc);
+        if (acc > 1) T.chunks.push_back(acc);
     }
     if (T.nbits) T.groups[T.nbits] = (uint32_t)T.chunks.size();
     return T;
@@ -141,6 +141,7 @@ static pfc_tuning default_tuning() {
     t.block_size = 256;
     t.walk_chunk = 1u << 16;
     t.seg_candidates = 1u << 22;
+    t.use_graph = 0;
     return t;
 }
 
@@ -200,6 +201,43 @@ extern "C" int pfc_get_tuning(const pfc_device* dev, pfc_tuning* t) {
     if (!dev || !t) return set_error(PFC_ERR_INVALID, "get_tuning: null pointer");
     *t = dev->tuning;
     return PFC_OK;
+}
+
+extern "C" int pfc_device_malloc(pfc_device* dev, size_t bytes, void** ptr) {
+    if (!dev || !ptr) return set_error(PFC_ERR_INVALID, "device_malloc: null pointer");
+    hipSetDevice(dev->ordinal);
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "hipMalloc");
+}
+
+extern "C" int pfc_device_free(pfc_device* dev, void* ptr) {
+    if (!dev) return set_error(PFC_ERR_INVALID, "device_free: null device");
+    hipSetDevice(dev->ordinal);
+    hipError_t e = hipFree(ptr);
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "hipFree");
+}
+
+extern "C" int pfc_memcpy(pfc_device* dev, void* dst, const void* src, size_t bytes, int kind) {
+    if (!dev || (!dst && bytes) || (!src && bytes)) return set_error(PFC_ERR_INVALID, "memcpy: null pointer");
+    if (kind < 1 || kind > 3) return set_error(PFC_ERR_INVALID, "memcpy: kind must be 1, 2 or 3");
+    hipSetDevice(dev->ordinal);
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost
+                                                                         : hipMemcpyDeviceToDevice;
+    hipError_t e = hipMemcpy(dst, src, bytes, k);
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "hipMemcpy");
+}
+
+extern "C" int pfc_device_synchronize(pfc_device* dev) {
+    if (!dev) return set_error(PFC_ERR_INVALID, "synchronize: null device");
+    hipSetDevice(dev->ordinal);
+    hipError_t e = hipDeviceSynchronize();
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "hipDeviceSynchronize");
+}
+
+extern "C" int pfc_probe_copy(pfc_device* dev, uint64_t bytes, double* ms) {
+    if (!dev) return set_error(PFC_ERR_INVALID, "probe_copy: null device");
+    hipSetDevice(dev->ordinal);
+    return pfc::probe_copy(bytes, ms);
 }
 
 extern "C" int pfc_probe_mul_rates(pfc_device* dev, double rates[3]) {
@@ -267,7 +305,7 @@ struct pfc_job {
     pfc_job_desc desc{};
     pfc_tuning tuning{};
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evw0 = nullptr, evw1 = nullptr;
     SegPlan plan{};
     uint64_t fcap = 0;
     uint64_t pos = 0;
@@ -276,8 +314,10 @@ struct pfc_job {
     std::vector<uint32_t> list;       // primorial multipliers
     std::vector<uint32_t> composite;  // bitmap over n - nmin (C and F|C)
     uint32_t jbegin = 0, jend = 0;    // walk step range
+    uint64_t steps_per_cand = 0;      // multiplier applications per candidate over the walk
     // device memory
-    uint32_t *d_ca = nullptr, *d_ga = nullptr, *d_cb = nullptr, *d_gb = nullptr;
+    uint64_t *d_ca = nullptr, *d_cb = nullptr;
+    uint32_t *d_ga = nullptr, *d_gb = nullptr;
     uint32_t* d_list = nullptr;
     uint32_t* d_comp = nullptr;
     uint64_t* d_p = nullptr;
@@ -286,9 +326,15 @@ struct pfc_job {
     pfc_ctrl* d_ctrl = nullptr;
     pfc_factor* d_fac = nullptr;
     pfc_ctrl* h_ctrl = nullptr;  // pinned
+    // captured segment pipeline (tuning.use_graph)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    uint64_t* d_bounds = nullptr;   // {lo, hi} read by the captured kernels
+    uint64_t* h_bounds = nullptr;   // pinned
     // results
     uint64_t dev_checksum = 0, primecount = 0, segments = 0;
-    double device_ms = 0;
+    double device_ms = 0, walk_ms = 0;
+    uint64_t walk_launches = 0, walk_updates = 0;
     std::vector<pfc_factor> factors;
 };
 
@@ -300,8 +346,14 @@ void free_job_device(pfc_job* j) {
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (j->h_ctrl) hipHostFree(j->h_ctrl);
+    if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
+    if (j->graph) (void)hipGraphDestroy(j->graph);
+    if (j->d_bounds) (void)hipFree(j->d_bounds);
+    if (j->h_bounds) (void)hipHostFree(j->h_bounds);
     if (j->ev0) hipEventDestroy(j->ev0);
     if (j->ev1) hipEventDestroy(j->ev1);
+    if (j->evw0) hipEventDestroy(j->evw0);
+    if (j->evw1) hipEventDestroy(j->evw1);
     if (j->stream) hipStreamDestroy(j->stream);
 }
 
@@ -330,21 +382,26 @@ int build_walk_data(pfc_job* j) {
         if (!last.empty()) return set_error(PFC_ERR_SIEVE, "CPU sieve failure.");
         j->jbegin = 0;
         j->jend = (uint32_t)j->list.size();
+        j->steps_per_cand = j->list.size();
         return PFC_OK;
     }
     j->jbegin = d.nmin;
     j->jend = d.nmax;
+    j->steps_per_cand = 0;
     if (d.modes & PFC_MODE_FACTORIAL) j->ta = build_table(0, N);
     if (d.modes & PFC_MODE_COMPOSITORIAL) {
         if (d.modes & PFC_MODE_FACTORIAL) j->tb = build_table(2, N);
         else j->ta = build_table(2, N);
         const uint32_t w = d.nmax - d.nmin;
         j->composite.assign(w / 32 + 2, 0xFFFFFFFFu);
-        for (uint32_t q : primes_in(d.nmin, d.nmax - 1)) {
+        const std::vector<uint32_t> pr = primes_in(d.nmin, d.nmax - 1);
+        for (uint32_t q : pr) {
             const uint32_t off = q - d.nmin;
             j->composite[off >> 5] &= ~(1u << (off & 31));
         }
+        j->steps_per_cand += w - pr.size();
     }
+    if (d.modes & PFC_MODE_FACTORIAL) j->steps_per_cand += d.nmax - d.nmin;
     return PFC_OK;
 }
 
@@ -397,6 +454,8 @@ extern "C" int pfc_job_create(pfc_device* dev, const pfc_job_desc* desc, pfc_job
     hipError_t e = hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&j->ev0);
     if (e == hipSuccess) e = hipEventCreate(&j->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&j->evw0);
+    if (e == hipSuccess) e = hipEventCreate(&j->evw1);
     if (e != hipSuccess) {
         rc = hip_fail(e, "stream/event create");
         free_job_device(j);
@@ -440,9 +499,65 @@ extern "C" int pfc_job_create(pfc_device* dev, const pfc_job_desc* desc, pfc_job
     return PFC_OK;
 }
 
+// Drains the control block and factor records of a finished segment.
+static int drain_segment(pfc_job* j, bool* overflow, uint64_t launches, bool timed);
+
+// Captures clear -> primes below 114 -> generate -> setup -> walk chunks ->
+// check for a job as one HIP graph; segment bounds come from device memory.
+static int build_graph(pfc_job* j) {
+    hipStream_t st = j->stream;
+    const uint32_t cap = j->plan.cap;
+    const pfc_job_desc& d = j->desc;
+    hipError_t e = hipMalloc((void**)&j->d_bounds, 2 * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&j->h_bounds, 2 * sizeof(uint64_t), hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "graph bounds");
+    e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    int rc = pfc::launch_head_dev(j->d_bounds, j->plan.span, j->d_p, cap, j->d_ctrl, st);
+    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits};
+    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits};
+    if (!rc)
+        rc = pfc::launch_setup(j->d_p, j->d_ctrl, j->d_v, j->d_one, j->d_ra, j->d_rb, &ta,
+                               j->d_rb ? &tb : nullptr, cap, st);
+    const uint32_t chunk = std::max<uint32_t>(j->tuning.walk_chunk, 1);
+    for (uint64_t a = j->jbegin; a < j->jend && !rc; a += chunk) {
+        const uint32_t b = (uint32_t)std::min<uint64_t>(j->jend, a + chunk);
+        rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, d.nmin, j->d_p, j->d_v,
+                              j->d_one, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
+                              cap, j->tuning.primes_per_lane, j->tuning.block_size, st);
+    }
+    if (!rc)
+        rc = pfc::launch_check(d.modes, d.nmax, j->d_p, j->d_v, j->d_one, j->d_ra, j->d_rb, j->d_ctrl,
+                               cap, st);
+    e = hipStreamEndCapture(st, &j->graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&j->gexec, j->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    return PFC_OK;
+}
+
+static int run_segment_graph(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
+    if (!j->gexec) {
+        int rc = build_graph(j);
+        if (rc) return rc;
+    }
+    hipStream_t st = j->stream;
+    j->h_bounds[0] = lo;
+    j->h_bounds[1] = hi;
+    hipError_t e = hipMemcpyAsync(j->d_bounds, j->h_bounds, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipGraphLaunch(j->gexec, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(j->h_ctrl, j->d_ctrl, sizeof(pfc_ctrl), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "graph segment");
+    return drain_segment(j, overflow, 0, false);
+}
+
 // One p-segment [lo, hi): generate, setup, walk in chunks, check, then drain
 // the control block and factor records (cl_sieve.cpp:1563-1726, 450-494).
 static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
+    if (j->tuning.use_graph) return run_segment_graph(j, lo, hi, overflow);
     hipStream_t st = j->stream;
     const uint32_t cap = j->plan.cap;
     const pfc_job_desc& d = j->desc;
@@ -457,19 +572,28 @@ static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
                            j->d_rb ? &tb : nullptr, cap, st);
     if (rc) return rc;
     const uint32_t chunk = std::max<uint32_t>(j->tuning.walk_chunk, 1);
-    for (uint64_t a = j->jbegin; a < j->jend; a += chunk) {
+    uint64_t launches = 0;
+    (void)hipEventRecord(j->evw0, st);
+    for (uint64_t a = j->jbegin; a < j->jend; a += chunk, ++launches) {
         const uint32_t b = (uint32_t)std::min<uint64_t>(j->jend, a + chunk);
         rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, d.nmin, j->d_p, j->d_v,
                               j->d_one, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
                               cap, j->tuning.primes_per_lane, j->tuning.block_size, st);
         if (rc) return rc;
     }
+    (void)hipEventRecord(j->evw1, st);
     rc = pfc::launch_check(d.modes, d.nmax, j->d_p, j->d_v, j->d_one, j->d_ra, j->d_rb, j->d_ctrl,
                            cap, st);
     if (rc) return rc;
     e = hipMemcpyAsync(j->h_ctrl, j->d_ctrl, sizeof(pfc_ctrl), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "segment");
+    return drain_segment(j, overflow, launches, true);
+}
+
+static int drain_segment(pfc_job* j, bool* overflow, uint64_t launches, bool timed) {
+    *overflow = false;
+    hipError_t e;
     const pfc_ctrl c = *j->h_ctrl;
     if (c.flags & 1u) {
         *overflow = true;
@@ -491,6 +615,13 @@ static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
     j->dev_checksum += c.checksum;
     j->primecount += c.primecount;
     j->segments += 1;
+    if (timed) {
+        float wms = 0;
+        (void)hipEventElapsedTime(&wms, j->evw0, j->evw1);
+        j->walk_ms += wms;
+        j->walk_launches += launches;
+        j->walk_updates += (uint64_t)c.ncand * j->steps_per_cand;
+    }
     return PFC_OK;
 }
 
@@ -545,6 +676,9 @@ extern "C" int pfc_job_stats_get(const pfc_job* j, pfc_job_stats* s) {
     s->raw_factors = j->factors.size();
     s->segments = j->segments;
     s->device_ms = j->device_ms;
+    s->walk_ms = j->walk_ms;
+    s->walk_launches = j->walk_launches;
+    s->walk_updates = j->walk_updates;
     return PFC_OK;
 }
 
@@ -856,7 +990,7 @@ extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** d
             continue;
         }
         const double per = (double)(j1 - j0) * ((j->d_rb) ? 1.9 : 1.0) +
-                           ((flags & 1u) ? 1.3 * (double)(j->ta.chunks.size() + j->tb.chunks.size()) : 0.0);
+                           ((flags & 1u) ? 2.2 * (double)(j->ta.chunks.size() + j->tb.chunks.size()) : 0.0);
         for (uint32_t f = 0; f < s.ncand; f += 64 * K) items.push_back({{(uint32_t)i, f, j0, j1, flags, {0, 0, 0}}, per});
         s.jcur = j1;
         if (j1 == j->jend) finishing[i] = 1;
@@ -1115,14 +1249,14 @@ extern "C" int pfc_parse_uint64(const char* s, uint64_t lo, uint64_t hi, uint64_
 
 // ====================================================================== tables (C ABI)
 
-extern "C" int pfc_build_setup_table(uint32_t kind, uint32_t N, uint32_t* chunks, size_t chunk_cap,
+extern "C" int pfc_build_setup_table(uint32_t kind, uint32_t N, uint64_t* chunks, size_t chunk_cap,
                                      size_t* nchunks, uint32_t* groups, size_t group_cap,
                                      uint32_t* nbits) {
     if (kind > 2 || !nchunks || !nbits) return set_error(PFC_ERR_INVALID, "build_setup_table: bad args");
     HostTable T = build_table(kind, N);
     *nchunks = T.chunks.size();
     *nbits = T.nbits;
-    if (chunks) memcpy(chunks, T.chunks.data(), std::min(chunk_cap, T.chunks.size()) * 4);
+    if (chunks) memcpy(chunks, T.chunks.data(), std::min(chunk_cap, T.chunks.size()) * 8);
     if (groups) memcpy(groups, T.groups.data(), std::min(group_cap, T.groups.size()) * 4);
     return PFC_OK;
 }

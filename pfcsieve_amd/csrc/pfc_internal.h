@@ -20,9 +20,9 @@ struct pfc_batch_job {
     pfc_ctrl* ctrl;
     pfc_factor* fac;
     uint32_t fcap, nba, nbb, pad;
-    const uint32_t* ca;
+    const uint64_t* ca;
     const uint32_t* ga;
-    const uint32_t* cb;
+    const uint64_t* cb;
     const uint32_t* gb;
     const uint32_t* list;
     const uint32_t* composite;
@@ -42,6 +42,8 @@ int hip_fail(hipError_t e, const char* what);
 
 int launch_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pfc_ctrl* d_ctrl,
                     hipStream_t st);
+int launch_head_dev(const void* d_bounds, uint64_t max_span, uint64_t* d_p, uint32_t cap,
+                    pfc_ctrl* d_ctrl, hipStream_t st);
 int launch_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uint64_t* d_one,
                  uint64_t* d_ra, uint64_t* d_rb, const pfc_setup_table* a,
                  const pfc_setup_table* b, uint32_t cap, hipStream_t st);
@@ -56,6 +58,7 @@ int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint3
 int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_item* d_items, uint32_t nitems,
                  uint32_t* d_head, uint32_t ppl, uint32_t nblocks, hipStream_t st);
 int probe_mul_rates(int cus, double rates[3]);
+int probe_copy(uint64_t bytes, double* ms);
 
 }  // namespace pfc
 

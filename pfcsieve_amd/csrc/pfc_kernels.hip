@@ -17,6 +17,8 @@
 
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "../../include/pfc.h"
 #include "pfc_arith.h"
 #include "pfc_internal.h"
@@ -54,9 +56,8 @@ __device__ __forceinline__ void wave_append(bool keep, uint64_t value, uint64_t*
 // the wheel and the 7..113 sieve are compacted in LDS, then PRP-tested by the
 // whole block (the reference's shape, getsegprimes.cl:477-517, with the LDS
 // list bounded by construction: at most 16 survivors per thread).
-__global__ __launch_bounds__(kGenBlock) void k_generate(uint64_t base30, uint64_t lo, uint64_t hi,
-                                                       uint64_t* out, uint32_t cap,
-                                                       pfc_ctrl* ctrl) {
+__device__ __forceinline__ void generate_block(uint64_t base30, uint64_t lo, uint64_t hi,
+                                               uint64_t* out, uint32_t cap, pfc_ctrl* ctrl) {
     __shared__ uint64_t s_list[kGenLds];
     __shared__ uint32_t s_mod[27];
     __shared__ uint32_t s_count;
@@ -93,10 +94,44 @@ __global__ __launch_bounds__(kGenBlock) void k_generate(uint64_t base30, uint64_
     }
 }
 
+__global__ __launch_bounds__(kGenBlock) void k_generate(uint64_t base30, uint64_t lo, uint64_t hi,
+                                                       uint64_t* out, uint32_t cap, pfc_ctrl* ctrl) {
+    generate_block(base30, lo, hi, out, cap, ctrl);
+}
+
+// Segment bounds in device memory, so that one captured graph serves every
+// p-segment of a job (the host writes the bounds before each replay).
+struct SegBounds {
+    uint64_t lo, hi;
+};
+
+__global__ __launch_bounds__(kGenBlock) void k_generate_dev(const SegBounds* sb, uint64_t* out,
+                                                           uint32_t cap, pfc_ctrl* ctrl) {
+    const uint64_t lo = sb->lo < 114 ? 114 : sb->lo;
+    const uint64_t hi = sb->hi;
+    if (hi <= lo) return;
+    generate_block(lo - lo % 30, lo, hi, out, cap, ctrl);
+}
+
+__global__ void k_zero_ctrl(pfc_ctrl* ctrl) {
+    if (threadIdx.x == 0) {
+        ctrl->ncand = 0;
+        ctrl->nfactor = 0;
+        ctrl->flags = 0;
+        ctrl->pad = 0;
+        ctrl->checksum = 0;
+        ctrl->primecount = 0;
+    }
+}
+
 // Primes below 114 in [lo, hi) (addsmallprimes.cl:25-38).  They are divisible
 // by a sieve prime, so k_generate never emits them.
 __global__ void k_small_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint32_t cap,
-                               pfc_ctrl* ctrl) {
+                               pfc_ctrl* ctrl, const SegBounds* sb) {
+    if (sb) {
+        lo = sb->lo;
+        hi = sb->hi < 114 ? sb->hi : 114;
+    }
     const uint32_t ps[29] = {3,  5,  7,  11, 13, 17, 19, 23, 29, 31,  37,  41,  43,  47, 53,
                              59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113};
     const uint32_t i = threadIdx.x;
@@ -111,15 +146,17 @@ __global__ void k_small_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint32_t
 // by R mod p so that it is the Montgomery form (X*R mod p) << s.
 // The tables are uniform across the wavefront and are read with scalar loads.
 typedef const uint32_t __attribute__((address_space(4)))* ConstU32;
+typedef const uint64_t __attribute__((address_space(4)))* ConstU64;
 
-__device__ __forceinline__ uint64_t horner_start(const uint32_t* chunks_, const uint32_t* groups_,
+__device__ __forceinline__ uint64_t horner_start(const uint64_t* chunks_, const uint32_t* groups_,
                                                  uint32_t nbits, const Modulus& m, uint64_t one) {
-    const ConstU32 chunks = (ConstU32)chunks_, groups = (ConstU32)groups_;
+    const ConstU64 chunks = (ConstU64)chunks_;
+    const ConstU32 groups = (ConstU32)groups_;
     uint64_t acc = 1ull << m.s;
     for (int b = (int)nbits - 1; b >= 0; --b) {
         if (b != (int)nbits - 1) acc = pfc::sqrmod(acc, m);
         const uint32_t g0 = groups[b], g1 = groups[b + 1];
-        for (uint32_t k = g0; k < g1; ++k) acc = pfc::mulmod32(acc, __builtin_amdgcn_readfirstlane(chunks[k]), m);
+        for (uint32_t k = g0; k < g1; ++k) acc = pfc::mulmod64(acc, chunks[k], m);
     }
     return pfc::mulmod64(acc, one >> m.s, m);
 }
@@ -128,9 +165,9 @@ __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
                                                const pfc_ctrl* __restrict__ ctrl,
                                                uint32_t* __restrict__ V, uint64_t* __restrict__ ONE,
                                                uint64_t* __restrict__ RA, uint64_t* __restrict__ RB,
-                                               const uint32_t* __restrict__ ca,
+                                               const uint64_t* __restrict__ ca,
                                                const uint32_t* __restrict__ ga, uint32_t nba,
-                                               const uint32_t* __restrict__ cb,
+                                               const uint64_t* __restrict__ cb,
                                                const uint32_t* __restrict__ gb, uint32_t nbb,
                                                uint32_t cap) {
     uint32_t n = ctrl->ncand;
@@ -477,6 +514,14 @@ __global__ __launch_bounds__(256) void k_probe(uint32_t* out, uint32_t c, int it
     out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
+// Plain copy of n 16-byte words: the reference traffic for HBM counters.
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                              uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
 // ================================================================== C entry points
@@ -500,7 +545,8 @@ int launch_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pfc_c
     if (hi <= lo) return PFC_OK;
     if (lo < 114) {
         const uint64_t h = hi < 114 ? hi : 114;
-        hipLaunchKernelGGL(k_small_primes, dim3(1), dim3(64), 0, st, lo, h, d_p, cap, d_ctrl);
+        hipLaunchKernelGGL(k_small_primes, dim3(1), dim3(64), 0, st, lo, h, d_p, cap, d_ctrl,
+                           (const SegBounds*)nullptr);
         lo = h;
         if (hi <= lo) return PFC_OK;
     }
@@ -513,6 +559,22 @@ int launch_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pfc_c
                        d_p, cap, d_ctrl);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_generate launch");
+}
+
+// The segment pipeline head for graph capture: clear the control block with
+// a kernel (no memset node), primes below 114, and the generator sized for
+// segments up to max_span integers, all reading bounds from d_bounds.
+int launch_head_dev(const void* d_bounds, uint64_t max_span, uint64_t* d_p, uint32_t cap,
+                    pfc_ctrl* d_ctrl, hipStream_t st) {
+    const SegBounds* sb = (const SegBounds*)d_bounds;
+    hipLaunchKernelGGL(k_zero_ctrl, dim3(1), dim3(64), 0, st, d_ctrl);
+    hipLaunchKernelGGL(k_small_primes, dim3(1), dim3(64), 0, st, 0ull, 0ull, d_p, cap, d_ctrl, sb);
+    const uint64_t per_block = (uint64_t)kGenBlock * kGenSpan;
+    const uint64_t blocks = (max_span + 60 + per_block - 1) / per_block;
+    if (blocks > 0x7FFFFFFFull) return set_error(PFC_ERR_CAPACITY, "segment too wide");
+    hipLaunchKernelGGL(k_generate_dev, dim3((uint32_t)blocks), dim3(kGenBlock), 0, st, sb, d_p, cap, d_ctrl);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "graph head launch");
 }
 
 int launch_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uint64_t* d_one,
@@ -598,6 +660,33 @@ int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_item* d_items, uin
     else return set_error(PFC_ERR_INVALID, "batch: primes_per_lane must be 1, 2 or 4");
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_batch launch");
+}
+
+int probe_copy(uint64_t bytes, double* ms_out) {
+    const uint64_t n = bytes / 16;
+    if (n == 0) return set_error(PFC_ERR_INVALID, "probe_copy: size below 16 bytes");
+    uint4 *a = nullptr, *b = nullptr;
+    hipError_t e = hipMalloc(&a, n * 16);
+    if (e == hipSuccess) e = hipMalloc(&b, n * 16);
+    if (e == hipSuccess) e = hipMemset(a, 1, n * 16);
+    hipEvent_t t0, t1;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    if (e == hipSuccess) {
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 1u << 16);
+        (void)hipEventRecord(t0, 0);
+        hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+        (void)hipEventRecord(t1, 0);
+        e = hipEventSynchronize(t1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        if (ms_out) *ms_out = ms;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "probe_copy");
 }
 
 int probe_mul_rates(int cus, double rates[3]) {

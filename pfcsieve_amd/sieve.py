@@ -38,8 +38,8 @@ class Device:
         check(lib().pfc_device_info(self._h, name, 128, C.byref(cus), C.byref(hbm)))
         return {"name": name.value.decode(), "compute_units": cus.value, "hbm_bytes": hbm.value}
 
-    def set_tuning(self, primes_per_lane=0, block_size=0, walk_chunk=0, seg_candidates=0):
-        t = Tuning(primes_per_lane, block_size, walk_chunk, seg_candidates)
+    def set_tuning(self, primes_per_lane=0, block_size=0, walk_chunk=0, seg_candidates=0, use_graph=0):
+        t = Tuning(primes_per_lane, block_size, walk_chunk, seg_candidates, use_graph)
         check(lib().pfc_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:
@@ -58,6 +58,11 @@ class Device:
         check(lib().pfc_probe_mul_rates(self._h, C.byref(r)))
         return {"mad_u64_u32": r[0], "mul_lo_u32": r[1], "mul_hi_u32": r[2]}
 
+    def probe_copy(self, nbytes: int) -> float:
+        ms = C.c_double()
+        check(lib().pfc_probe_copy(self._h, nbytes, C.byref(ms)))
+        return ms.value
+
     def close(self):
         if self._h:
             lib().pfc_device_close(self._h)
@@ -66,6 +71,40 @@ class Device:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """A raw device allocation (pfc_device_malloc) for stage-level calls."""
+
+    def __init__(self, dev: Device, nbytes: int):
+        self.dev, self.nbytes = dev, nbytes
+        p = C.c_void_p()
+        check(lib().pfc_device_malloc(dev.handle, nbytes, C.byref(p)))
+        self.ptr = p
+
+    def upload(self, data: bytes):
+        buf = C.create_string_buffer(bytes(data), len(data))
+        check(lib().pfc_memcpy(self.dev.handle, self.ptr, buf, len(data), 1))
+
+    def download(self, nbytes: Optional[int] = None) -> bytes:
+        n = self.nbytes if nbytes is None else nbytes
+        buf = C.create_string_buffer(n)
+        check(lib().pfc_memcpy(self.dev.handle, buf, self.ptr, n, 2))
+        return buf.raw
+
+    def zero(self):
+        self.upload(b"\0" * self.nbytes)
+
+    def free(self):
+        if self.ptr:
+            lib().pfc_device_free(self.dev.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass
 

@@ -35,7 +35,7 @@ def test_batch_matches_oracle(orc, dev, max_steps):
 
 def test_batch_multi_segment_jobs(orc, dev):
     import pfcsieve_amd as PK
-    dev.set_tuning(seg_candidates=3000)
+    dev.set_tuning(seg_candidates=1000)
     try:
         got = PK.run_batch(MIX[:4], device=dev, max_steps=1500, verify_threads=8)
     finally:

@@ -12,30 +12,25 @@ F, P, Cm = 1, 2, 4
 TWO64 = 1 << 64
 
 
-def _torch():
-    import torch
-    assert torch.cuda.is_available(), "GPU tests need a HIP device"
-    return torch
-
-
-def _ptr(t):
-    return C.c_void_p(t.data_ptr())
-
-
-def gen_on_gpu(lo, hi, cap=None):
-    torch = _torch()
+def gen_on_gpu(dev, lo, hi, cap=None):
+    """Stage (a) alone through the C ABI, on library-allocated device memory."""
     import pfcsieve_amd._native as N
+    from pfcsieve_amd.sieve import DeviceBuffer
     span = hi - lo
     if cap is None:
         cap = int(2.2 * span / max(np.log(max(span, 3)), 1.0)) + 4096
-    d_p = torch.zeros(cap, dtype=torch.int64, device="cuda")
-    d_ctrl = torch.zeros(4, dtype=torch.int64, device="cuda")
-    N.check(N.lib().pfc_stage_generate(lo, hi, _ptr(d_p), cap, _ptr(d_ctrl), None))
-    torch.cuda.synchronize()
-    ctrl = d_ctrl.cpu().numpy().view(np.uint32)
+    d_p = DeviceBuffer(dev, cap * 8)
+    d_ctrl = DeviceBuffer(dev, 32)
+    d_ctrl.zero()
+    N.check(N.lib().pfc_stage_generate(lo, hi, d_p.ptr, cap, d_ctrl.ptr, None))
+    N.check(N.lib().pfc_device_synchronize(dev.handle))
+    ctrl = np.frombuffer(d_ctrl.download(), dtype=np.uint32)
     n = int(ctrl[0])
     assert ctrl[2] & 1 == 0, "candidate overflow"
-    return np.sort(d_p[:n].cpu().numpy().view(np.uint64)), d_p, d_ctrl, cap
+    got = np.sort(np.frombuffer(d_p.download(8 * n), dtype=np.uint64)) if n else np.zeros(0, np.uint64)
+    d_p.free()
+    d_ctrl.free()
+    return got
 
 
 @pytest.mark.parametrize("lo,hi", [
@@ -47,15 +42,15 @@ def gen_on_gpu(lo, hi, cap=None):
     ((1 << 62) - 4 * 10**6, 1 << 62),  # cfg3 carry edge, scaled down
     (TWO64 - 2 * 10**6, TWO64 - 1),    # top of the 64-bit range
 ])
-def test_generate_matches_oracle(orc, lo, hi):
-    got, *_ = gen_on_gpu(lo, hi)
+def test_generate_matches_oracle(orc, dev, lo, hi):
+    got = gen_on_gpu(dev, lo, hi)
     ref = orc.candidates(lo, hi)
     assert len(got) == len(ref)
     assert np.array_equal(got, np.sort(ref))
 
 
-def test_generate_cfg1_count(orc):
-    got, *_ = gen_on_gpu(10**9, 10**9 + 10**6)
+def test_generate_cfg1_count(orc, dev):
+    got = gen_on_gpu(dev, 10**9, 10**9 + 10**6)
     assert len(got) == 48155
 
 
@@ -136,3 +131,100 @@ def test_reference_selftest_vector(dev, golden, case):
     assert got.factorcount == g["factorcount"]
     assert got.primecount == g["primecount"]
     assert f"{got.checksum:016X}" == g["checksum"]
+
+
+@pytest.mark.parametrize("job", [(F | Cm, 101, 200000, 101, 3000), (P, 10**9, 10**9 + 4 * 10**5, 2000, 6000),
+                                 (Cm, 10**12, 10**12 + 3 * 10**5, 101, 2500)])
+def test_graph_replay_matches_eager(orc, dev, job):
+    """The captured segment pipeline, replayed once per segment (many
+    replays, with eager jobs run between them), gives the oracle's result."""
+    import pfcsieve_amd as PK
+    ref = orc.run(*job)
+    dev.set_tuning(seg_candidates=2000, walk_chunk=900, use_graph=1)
+    try:
+        g = PK.Job(dev, *job)
+        k = 0
+        while g.position() < job[2]:
+            g.run(1)
+            k += 1
+            if k % 3 == 0:   # eager work between replays
+                dev.set_tuning(seg_candidates=2000, walk_chunk=900, use_graph=0)
+                e = PK.run_sieve(F, 10**6, 10**6 + 5000, 101, 600, device=dev)
+                r2 = orc.run(F, 10**6, 10**6 + 5000, 101, 600)
+                assert e.checksum == r2.checksum
+                dev.set_tuning(seg_candidates=2000, walk_chunk=900, use_graph=1)
+        got = PK.job_result(g, verify_threads=8)
+        g.destroy()
+    finally:
+        dev.set_tuning()
+    assert k > 3
+    assert (got.checksum, got.primecount, got.factorcount) == (ref.checksum, ref.primecount, ref.factorcount)
+
+
+def _table(dev, kind, N):
+    """pfc_build_setup_table on the host, uploaded to device buffers."""
+    import pfcsieve_amd._native as Nt
+    from pfcsieve_amd.sieve import DeviceBuffer
+    L = Nt.lib()
+    nch, nb = C.c_size_t(), C.c_uint32()
+    Nt.check(L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb)))
+    ch = (C.c_uint64 * max(nch.value, 1))()
+    gr = (C.c_uint32 * (nb.value + 1))()
+    Nt.check(L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb)))
+    dc = DeviceBuffer(dev, 8 * max(nch.value, 1))
+    dc.upload(bytes(ch))
+    dg = DeviceBuffer(dev, 4 * (nb.value + 1))
+    dg.upload(bytes(gr))
+    return Nt.SetupTable(dc.ptr.value, dg.ptr.value, nb.value), (dc, dg)
+
+
+@pytest.mark.parametrize("modes,nmin,nmax", [(F, 2000, 2600), (Cm, 2000, 2600), (F | Cm, 2000, 2600)])
+def test_stages_individually(orc, dev, modes, nmin, nmax):
+    """generate -> setup -> walk (two launches) -> check, each called through
+    the stage ABI; residues after setup and the checksum against the oracle."""
+    import pfcsieve_amd._native as Nt
+    from pfcsieve_amd.sieve import DeviceBuffer
+    L = Nt.lib()
+    lo, hi = 10**10, 10**10 + 50000
+    cap = 8192
+    bufs = {k: DeviceBuffer(dev, cap * w) for k, w in
+            [("p", 8), ("v", 4), ("one", 8), ("ra", 8), ("rb", 8)]}
+    ctrl = DeviceBuffer(dev, 32)
+    ctrl.zero()
+    fac = DeviceBuffer(dev, 16 * 1024)
+    Nt.check(L.pfc_stage_generate(lo, hi, bufs["p"].ptr, cap, ctrl.ptr, None))
+    two = modes == (F | Cm)
+    ta, keep_a = _table(dev, 0 if modes & F else 2, nmin - 1)
+    tb, keep_b = _table(dev, 2, nmin - 1)
+    Nt.check(L.pfc_stage_setup(bufs["p"].ptr, ctrl.ptr, bufs["v"].ptr, bufs["one"].ptr, bufs["ra"].ptr,
+                               bufs["rb"].ptr if two else None, C.byref(ta), C.byref(tb) if two else None,
+                               cap, None))
+    Nt.check(L.pfc_device_synchronize(dev.handle))
+    n = int(np.frombuffer(ctrl.download(), np.uint32)[0])
+    p = np.frombuffer(bufs["p"].download(8 * n), np.uint64)
+    ra = np.frombuffer(bufs["ra"].download(8 * n), np.uint64)
+    for i in range(0, n, max(1, n // 50)):
+        pi = int(p[i])
+        rec, _ = orc.candidate_record(modes, pi, nmin, nmin)    # state after setup, no walk
+        s = 64 - pi.bit_length()
+        want = rec[6] if modes & F else rec[6]
+        assert int(ra[i]) >> s == want
+    # composite bitmap over [nmin, nmax)
+    comp = np.zeros((nmax - nmin) // 32 + 2, np.uint32)
+    isp = set(int(x) for x in orc.primes(nmin, nmax - 1))
+    for k in range(nmax - nmin):
+        if nmin + k not in isp:
+            comp[k >> 5] |= np.uint32(1 << (k & 31))
+    dcomp = DeviceBuffer(dev, comp.nbytes)
+    dcomp.upload(comp.tobytes())
+    mid = (nmin + nmax) // 2
+    for j0, j1 in [(nmin, mid), (mid, nmax)]:
+        Nt.check(L.pfc_stage_walk(modes, j0, j1, None, dcomp.ptr if modes & Cm else None, nmin, bufs["p"].ptr,
+                                  bufs["v"].ptr, bufs["one"].ptr, bufs["ra"].ptr, bufs["rb"].ptr if two else None,
+                                  ctrl.ptr, fac.ptr, 1024, cap, None))
+    Nt.check(L.pfc_stage_check(modes, nmax, bufs["p"].ptr, bufs["v"].ptr, bufs["one"].ptr, bufs["ra"].ptr,
+                               bufs["rb"].ptr if two else None, ctrl.ptr, cap, None))
+    Nt.check(L.pfc_device_synchronize(dev.handle))
+    c = np.frombuffer(ctrl.download(), np.uint64)
+    ref = orc.run(modes, lo, hi, nmin, nmax)
+    assert int(c[2]) == ref.dev_checksum and int(c[3]) == ref.primecount == n

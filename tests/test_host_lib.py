@@ -121,7 +121,7 @@ def test_verify_factor(L):
 def test_setup_table_reconstructs_start_value(L, kind, N):
     nch, nb = C.c_size_t(), C.c_uint32()
     L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb))
-    ch = (C.c_uint32 * nch.value)()
+    ch = (C.c_uint64 * nch.value)()
     gr = (C.c_uint32 * (nb.value + 1))()
     assert L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb)) == 0
     x = 1
@@ -132,7 +132,7 @@ def test_setup_table_reconstructs_start_value(L, kind, N):
     prim = sympy.primorial(N, nth=False)
     expect = [math.factorial(N), prim, math.factorial(N) // prim][kind]
     assert x == expect
-    assert all(c < 2**32 for c in ch)
+    assert all(1 < c < 2**64 for c in ch)
 
 
 # ----------------------------------------------------------------- device arithmetic on the host
