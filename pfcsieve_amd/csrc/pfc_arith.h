@@ -32,6 +32,20 @@
 
 namespace pfc {
 
+// 32-bit subtract with borrow in/out.
+PFC_HD uint32_t subb32(uint32_t a, uint32_t b, uint32_t bin, uint32_t* bout) {
+#if defined(__clang__)
+    unsigned int o;
+    const uint32_t r = __builtin_subc(a, b, bin, &o);
+    *bout = o;
+    return r;
+#else
+    const uint64_t r = (uint64_t)a - b - bin;
+    *bout = (uint32_t)(r >> 63);
+    return (uint32_t)r;
+#endif
+}
+
 struct Modulus {
     uint32_t d1, d0;  // d = p << s, high and low words (d1 >= 2^31)
     uint32_t v;       // 3-by-2 reciprocal of d
@@ -78,12 +92,24 @@ PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t d1, uint32_t d0,
     const uint64_t qq = (uint64_t)v * u2 + hi;
     const uint32_t q1 = (uint32_t)(qq >> 32), q0 = (uint32_t)qq;
     const uint64_t t = (uint64_t)d0 * q1 + d;
-    const uint32_t r1 = u1 - q1 * d1;
-    const uint32_t t0 = (uint32_t)t;
-    const uint32_t xlo = u0 - t0;
-    const uint32_t xhi = r1 - (uint32_t)(t >> 32) - (uint32_t)(u0 < t0);
+    // <u1 - q1*d1, u0> - t with an explicit borrow chain across the halves
+    // (v_sub_co / v_subb_co on the device).
+    uint32_t borrow, borrow2;
+    const uint32_t xlo = subb32(u0, (uint32_t)t, 0u, &borrow);
+    const uint32_t xhi = subb32(u1 - q1 * d1, (uint32_t)(t >> 32), borrow, &borrow2);
+    (void)borrow2;
     uint64_t x = ((uint64_t)xhi << 32) | xlo;
+#if defined(__HIP_DEVICE_COMPILE__)
+    // An exec-masked add instead of two selects: the empty volatile asm keeps
+    // the compiler from turning the branch into v_cndmask pairs, so the
+    // correction costs one VALU add plus scalar exec-mask bookkeeping.
+    if (xhi >= q0) {
+        asm volatile("" ::: );
+        x += d;
+    }
+#else
     if (xhi >= q0) x += d;
+#endif
     *fix = x >= d;
     return x;
 }

@@ -145,7 +145,9 @@ def A():
     hdr = os.path.join(ROOT, "pfcsieve_amd", "csrc", "pfc_arith.h")
     if not os.path.exists(out) or os.path.getmtime(out) < max(os.path.getmtime(src), os.path.getmtime(hdr)):
         os.makedirs(os.path.dirname(out), exist_ok=True)
-        subprocess.run(["g++", "-O2", "-fPIC", "-shared", "-o", out, src], check=True)
+        # clang, as on the device (the same builtins are compiled)
+        cxx = "/opt/rocm/llvm/bin/clang++" if os.path.exists("/opt/rocm/llvm/bin/clang++") else "g++"
+        subprocess.run([cxx, "-O2", "-fPIC", "-shared", "-o", out, src], check=True)
     L = C.CDLL(out)
     u64, u32 = C.c_uint64, C.c_uint32
     L.t_modulus.argtypes = [u64, C.POINTER(u32 * 4)]
