@@ -90,6 +90,34 @@ def mix_compare(dev, units, verify_threads=16):
             "rounds": rounds, "results_identical": sig_plain == sig_batch}
 
 
+def serve_run(dev, units, max_steps=50_000):
+    """Served stream: the units arrive as a seeded Poisson process at 80% of
+    the batch capacity measured first on the same units; admission by HBM
+    footprint; reports throughput and p50/p99 job latency."""
+    import pfcsieve_amd as P
+    import pfcsieve_amd.serve as S
+    from pfcsieve_amd import workload as W
+    steps = [W.walk_steps(u) for u in units]
+    t = time.perf_counter()
+    res = P.run_batch([u.desc() for u in units], device=dev, verify_threads=16)
+    cap_s = time.perf_counter() - t
+    rate = 0.8 * len(units) / cap_s
+    arrivals = W.poisson_arrivals(len(units), rate, seed=777)
+    info = dev.info()
+    srv = S.Server(dev, hbm_budget=int(info["hbm_bytes"] * 0.8), max_steps=max_steps, verify_threads=16)
+    for k, (u, a) in enumerate(zip(units, arrivals)):
+        srv.submit(S.Request(k, u.desc(), a))
+    st = srv.run()
+    by_id = {r.rid: r for r in srv.done}
+    same = all((by_id[k].result.checksum, by_id[k].result.primecount) == (res[k].checksum, res[k].primecount)
+               for k in range(len(units)))
+    upd = sum(r.primecount * s for r, s in zip(res, steps))
+    return {"jobs": st.jobs, "arrival_rate_per_s": rate, "wall_s": st.wall_s, "rounds": st.rounds,
+            "updates_per_s": upd / st.wall_s, "p50_latency_s": st.pct(50), "p99_latency_s": st.pct(99),
+            "peak_resident_bytes": st.peak_resident_bytes, "results_equal_batch": same,
+            "max_steps_per_round": max_steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +131,8 @@ def main():
     ap.add_argument("--mix-units", type=int, default=24,
                     help="units of the seeded 256-unit batched mix to compare plain vs batched (0 = skip)")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=0,
+                    help="jobs of the seeded served stream to run through the serving loop (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run one plain copy of this many bytes first (HBM counter calibration)")
     args = ap.parse_args()
@@ -174,6 +204,13 @@ def main():
         line["batched"]["mix"] = ("first %d of 256 seeded units (splitmix64 seed 12345): modes F/P/C equal odds, "
                                   "n-width log-U[1e3,1e6] from 101+U[0,1e6], p-width log-U[1e6,1e9] from "
                                   "log-U[1e10,2^63]" % args.mix_units)
+    if args.serve_jobs:
+        from pfcsieve_amd import workload as W
+        units = W.mix(10_000, seed=2024)[: args.serve_jobs]
+        line["served"] = serve_run(dev, units)
+        line["served"]["stream"] = ("first %d of 10,000 seeded jobs (splitmix64 seed 2024, same unit distribution "
+                                    "as the batched mix), Poisson arrivals at 0.8x measured batch capacity"
+                                    % args.serve_jobs)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(modes, pmin, args.cpu_width, nmin, nmax, steps_per_cand)
         line["cpu_baseline"]["gpu_over_cpu"] = ups / line["cpu_baseline"]["value"]

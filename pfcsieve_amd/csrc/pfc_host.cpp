@@ -137,7 +137,7 @@ struct pfc_device {
 
 static pfc_tuning default_tuning() {
     pfc_tuning t;
-    t.primes_per_lane = 2;
+    t.primes_per_lane = 1;   // fastest walk shape measured on MI355X (pfc_autotune)
     t.block_size = 256;
     t.walk_chunk = 1u << 16;
     t.seg_candidates = 1u << 22;
