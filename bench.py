@@ -118,6 +118,27 @@ def serve_run(dev, units, max_steps=50_000):
             "max_steps_per_round": max_steps}
 
 
+def verify_baseline(dev):
+    """The CPU factor verification (getResults' OpenMP verify, cl_sieve.cpp:509-540)
+    on the factor records of self-test case 3 (-! p in [101, 1e5), n in [101, 1e6),
+    42,821 factors), at 2 threads (the -v default) and at all host threads."""
+    import ctypes as C
+    import pfcsieve_amd as P
+    from pfcsieve_amd._native import check, lib
+    j = P.Job(dev, 1, 101, 100000, 101, 1000000)
+    j.run()
+    arr, n = j.raw_factors()
+    j.destroy()
+    out = {"factors": n, "sample": "self-test case 3 factor records"}
+    for threads in (2, os.cpu_count() or 2):
+        cp = (type(arr[0]) * n)(*arr[:n])
+        cs, kept = C.c_uint64(0), C.c_size_t()
+        t = time.perf_counter()
+        check(lib().pfc_harvest(cp, n, threads, C.byref(cs), C.byref(kept)))
+        out[f"threads_{threads}_s"] = time.perf_counter() - t
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +235,7 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(modes, pmin, args.cpu_width, nmin, nmax, steps_per_cand)
         line["cpu_baseline"]["gpu_over_cpu"] = ups / line["cpu_baseline"]["value"]
+        line["cpu_baseline"]["verify"] = verify_baseline(dev)
     dev.close()
     print(json.dumps(line), flush=True)
 

@@ -283,12 +283,29 @@ struct WalkStream {
 template <int MODE, int K>
 __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K], const Lanes<K>& L,
                                            uint32_t j0, uint32_t j1, const WalkStream& W) {
-    if (MODE == 0 || MODE == 1) {
-        const int32_t type = MODE == 1 ? PFC_PRIMORIAL : PFC_FACTORIAL;
-        const ConstU32 list = (ConstU32)W.list;
+    if (MODE == 0) {
         for (uint32_t j = j0; j < j1; ++j) {
-            const uint32_t n = __builtin_amdgcn_readfirstlane(MODE == 1 ? list[j] : j);
-            if (step<K>(ra, n, L)) record_hits<K>(ra, n, type, L, W.fac, W.fcap, W.ctrl);
+            const uint32_t n = __builtin_amdgcn_readfirstlane(j);
+            if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_FACTORIAL, L, W.fac, W.fcap, W.ctrl);
+        }
+    } else if (MODE == 1) {
+        // The primes are fetched 16 at a time (one scalar block load) so the
+        // wavefront does not wait on a scalar load every step.
+        const ConstU32 list = (ConstU32)W.list;
+        uint32_t j = j0;
+        for (; j + 16 <= j1; j += 16) {
+            uint32_t q[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) q[t] = list[j + t];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const uint32_t n = __builtin_amdgcn_readfirstlane(q[t]);
+                if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_PRIMORIAL, L, W.fac, W.fcap, W.ctrl);
+            }
+        }
+        for (; j < j1; ++j) {
+            const uint32_t n = __builtin_amdgcn_readfirstlane(list[j]);
+            if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_PRIMORIAL, L, W.fac, W.fcap, W.ctrl);
         }
     } else {
         const ConstU32 comp = (ConstU32)W.composite;
