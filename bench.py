@@ -139,6 +139,34 @@ def verify_baseline(dev):
     return out
 
 
+def other_configs(dev):
+    """Throughput on the other single-GPU configurations: cfg1 (the reference's
+    CPU-runnable case, whole job) and cfg3 (primorial near 2^62, one p-segment)."""
+    import pfcsieve_amd as P
+    out = {}
+    # cfg1: -! -n 101 -N 5000 -p 1e9 -P 1.001e9 (48,155 candidates)
+    t = time.perf_counter()
+    r = P.run_sieve(1, 10**9, 10**9 + 10**6, 101, 5000, device=dev, verify_threads=16)
+    dt = time.perf_counter() - t
+    out["cfg1"] = {"workload": "-! n [101,5000) p [1e9,1.001e9)", "candidates": r.primecount,
+                   "updates": r.primecount * 4899, "wall_s": dt, "updates_per_s": r.primecount * 4899 / dt,
+                   "factors": r.factorcount, "checksum": "%016X" % r.checksum}
+    # cfg3: -# -n 101 -N 1e7 -p 2^62-4e9 -P 2^62, one segment
+    from pfcsieve_amd import workload as W
+    steps = W.walk_steps(W.Unit(2, 0, 0, 101, 10**7))
+    j = P.Job(dev, 2, 2**62 - 4 * 10**9, 2**62, 101, 10**7)
+    t = time.perf_counter()
+    j.run(1)
+    dt = time.perf_counter() - t
+    s = j.stats()
+    j.destroy()
+    out["cfg3_segment"] = {"workload": "-# n [101,1e7) p from 2^62-4e9, one p-segment", "candidates": s.primecount,
+                           "steps_per_candidate": steps, "wall_s": dt,
+                           "updates_per_s": s.primecount * steps / dt,
+                           "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +180,7 @@ def main():
     ap.add_argument("--mix-units", type=int, default=24,
                     help="units of the seeded 256-unit batched mix to compare plain vs batched (0 = skip)")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--configs", action="store_true", help="also measure cfg1 and one cfg3 segment")
     ap.add_argument("--serve-jobs", type=int, default=0,
                     help="jobs of the seeded served stream to run through the serving loop (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -218,6 +247,8 @@ def main():
                              f"(mad_u64_u32 {rates['mad_u64_u32']:.3e}, mul_lo {rates['mul_lo_u32']:.3e}, "
                              f"mul_hi {rates['mul_hi_u32']:.3e})"},
     }
+    if args.configs:
+        line["other_configs"] = other_configs(dev)
     if args.mix_units:
         from pfcsieve_amd import workload as W
         units = W.mix(256, seed=12345)[: args.mix_units]
