@@ -221,6 +221,61 @@ PFC_HD bool strong_prp2(uint64_t N) {
     return false;
 }
 
+// ---------------------------------------------------------------- Montgomery (start residues)
+
+// p^-1 mod 2^64 for odd p (Newton: each step doubles the correct low bits;
+// x = p is right to 3 bits because p*p == 1 mod 8).
+PFC_HD uint64_t inv64(uint64_t p) {
+    uint64_t x = p;
+    for (int i = 0; i < 5; ++i) x *= 2 - p * x;
+    return x;
+}
+
+// Montgomery product a*b*2^-64 mod p, canonical in [0, p) when a*b < p*2^64
+// (the reduction of kernels/iterate.cl:44-50, written with 32x32->64 pieces:
+// 4 products for a*b, 3 for m = lo(a*b)*q, 4 for hi(m*p)).
+PFC_HD uint64_t redc_mul(uint64_t a, uint64_t b, uint64_t p, uint64_t q) {
+    const uint64_t a0 = (uint32_t)a, a1 = a >> 32, b0 = (uint32_t)b, b1 = b >> 32;
+    const uint64_t p00 = a0 * b0;
+    const uint64_t p01 = a0 * b1 + (p00 >> 32);
+    const uint64_t p10 = a1 * b0 + (uint32_t)p01;
+    const uint64_t hi = a1 * b1 + (p01 >> 32) + (p10 >> 32);
+    const uint64_t lo = (p10 << 32) | (uint32_t)p00;
+    const uint64_t m = lo * q;
+    const uint64_t mp = mulhi64(m, p);
+    const uint64_t r = hi - mp;
+    return hi < mp ? r + p : r;
+}
+
+// Start residue (X * 2^64 mod p) << s of X = prod over groups b (high first)
+// of (prod of chunks in group b)^(2^b), by Horner's rule in Montgomery
+// arithmetic without converting the chunks: every product divides by 2^64,
+// so the accumulator ends as X * 2^(64E) with an exponent E < 1 fixed by the
+// table alone; the host passes k = 1 - E and one product with (R^2)^k
+// computed by powering (R^2 mod p is 1 * R^2 in Montgomery terms, and its
+// k-th power is R^(k+1) = R^(2-E)) restores the Montgomery form X * R.
+template <class Chunks, class Groups>
+PFC_HD uint64_t start_residue(Chunks chunks, Groups groups, uint32_t nbits, uint64_t k,
+                              const Modulus& m, uint64_t one_shifted) {
+    const uint64_t p = mod_d(m) >> m.s;
+    const uint64_t q = inv64(p);
+    uint64_t acc = 1;
+    for (int b = (int)nbits - 1; b >= 0; --b) {
+        if (b != (int)nbits - 1) acc = redc_mul(acc, acc, p, q);
+        const uint32_t g0 = groups[b], g1 = groups[b + 1];
+        for (uint32_t i = g0; i < g1; ++i) acc = redc_mul(acc, chunks[i], p, q);
+    }
+    // R^2 mod p from R mod p, then its k-th power by left-to-right squaring.
+    const uint64_t one = one_shifted >> m.s;
+    const uint64_t r2 = mulmod64(one_shifted, one, m) >> m.s;
+    uint64_t f = r2;
+    for (int bit = 62 - (int)clz64(k); bit >= 0; --bit) {
+        f = redc_mul(f, f, p, q);
+        if ((k >> bit) & 1) f = redc_mul(f, r2, p, q);
+    }
+    return redc_mul(acc, f, p, q) << m.s;
+}
+
 // ---------------------------------------------------------------- wheel + small-prime sieve
 
 // The sieve primes of the reference's candidate generator (kernels/

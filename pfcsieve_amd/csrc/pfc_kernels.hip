@@ -148,17 +148,10 @@ __global__ void k_small_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint32_t
 typedef const uint32_t __attribute__((address_space(4)))* ConstU32;
 typedef const uint64_t __attribute__((address_space(4)))* ConstU64;
 
-__device__ __forceinline__ uint64_t horner_start(const uint64_t* chunks_, const uint32_t* groups_,
-                                                 uint32_t nbits, const Modulus& m, uint64_t one) {
-    const ConstU64 chunks = (ConstU64)chunks_;
-    const ConstU32 groups = (ConstU32)groups_;
-    uint64_t acc = 1ull << m.s;
-    for (int b = (int)nbits - 1; b >= 0; --b) {
-        if (b != (int)nbits - 1) acc = pfc::sqrmod(acc, m);
-        const uint32_t g0 = groups[b], g1 = groups[b + 1];
-        for (uint32_t k = g0; k < g1; ++k) acc = pfc::mulmod64(acc, chunks[k], m);
-    }
-    return pfc::mulmod64(acc, one >> m.s, m);
+__device__ __forceinline__ uint64_t horner_start(const uint64_t* chunks, const uint32_t* groups,
+                                                 uint32_t nbits, uint64_t k, const Modulus& m,
+                                                 uint64_t one) {
+    return pfc::start_residue((ConstU64)chunks, (ConstU32)groups, nbits, k, m, one);
 }
 
 __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
@@ -167,9 +160,9 @@ __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
                                                uint64_t* __restrict__ RA, uint64_t* __restrict__ RB,
                                                const uint64_t* __restrict__ ca,
                                                const uint32_t* __restrict__ ga, uint32_t nba,
-                                               const uint64_t* __restrict__ cb,
+                                               uint64_t ka, const uint64_t* __restrict__ cb,
                                                const uint32_t* __restrict__ gb, uint32_t nbb,
-                                               uint32_t cap) {
+                                               uint64_t kb, uint32_t cap) {
     uint32_t n = ctrl->ncand;
     if (n > cap) n = cap;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -178,8 +171,8 @@ __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
         const uint64_t one = pfc::mont_one(m);
         V[i] = m.v;
         ONE[i] = one;
-        RA[i] = horner_start(ca, ga, nba, m, one);
-        if (RB) RB[i] = horner_start(cb, gb, nbb, m, one);
+        RA[i] = horner_start(ca, ga, nba, ka, m, one);
+        if (RB) RB[i] = horner_start(cb, gb, nbb, kb, m, one);
     }
 }
 
@@ -445,8 +438,8 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
             const Modulus m = pfc::make_modulus(p);
             const uint64_t one = pfc::mont_one(m);
             lanes_from_state<K>(L, p, m.v, one, k);
-            ra[k] = horner_start(J.ca, J.ga, J.nba, m, one);
-            rb[k] = two ? horner_start(J.cb, J.gb, J.nbb, m, one) : 0;
+            ra[k] = horner_start(J.ca, J.ga, J.nba, J.ka, m, one);
+            rb[k] = two ? horner_start(J.cb, J.gb, J.nbb, J.kb, m, one) : 0;
         } else {
             lanes_from_state<K>(L, p, J.V[src], J.ONE[src], k);
             ra[k] = J.RA[src];
@@ -603,8 +596,8 @@ int launch_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uin
     if (blocks == 0) blocks = 1;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, st, d_p, d_ctrl, d_v, d_one, d_ra, d_rb,
-                       a->chunks, a->groups, a->nbits, b ? b->chunks : nullptr,
-                       b ? b->groups : nullptr, b ? b->nbits : 0u, cap);
+                       a->chunks, a->groups, a->nbits, a->mont_k, b ? b->chunks : nullptr,
+                       b ? b->groups : nullptr, b ? b->nbits : 0u, b ? b->mont_k : 1ull, cap);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_setup launch");
 }

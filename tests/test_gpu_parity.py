@@ -166,16 +166,17 @@ def _table(dev, kind, N):
     import pfcsieve_amd._native as Nt
     from pfcsieve_amd.sieve import DeviceBuffer
     L = Nt.lib()
-    nch, nb = C.c_size_t(), C.c_uint32()
-    Nt.check(L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb)))
+    nch, nb, mk = C.c_size_t(), C.c_uint32(), C.c_uint64()
+    Nt.check(L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb), C.byref(mk)))
     ch = (C.c_uint64 * max(nch.value, 1))()
     gr = (C.c_uint32 * (nb.value + 1))()
-    Nt.check(L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb)))
+    Nt.check(L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb),
+                                     C.byref(mk)))
     dc = DeviceBuffer(dev, 8 * max(nch.value, 1))
     dc.upload(bytes(ch))
     dg = DeviceBuffer(dev, 4 * (nb.value + 1))
     dg.upload(bytes(gr))
-    return Nt.SetupTable(dc.ptr.value, dg.ptr.value, nb.value), (dc, dg)
+    return Nt.SetupTable(dc.ptr.value, dg.ptr.value, nb.value, mk.value), (dc, dg)
 
 
 @pytest.mark.parametrize("modes,nmin,nmax", [(F, 2000, 2600), (Cm, 2000, 2600), (F | Cm, 2000, 2600)])
