@@ -24,6 +24,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 W_MUL = 19  # INT32 multiply instructions per update in the reference formulation
+# HBM bytes per candidate per walk launch, from the FETCH_SIZE (x2.000 copy-kernel
+# calibration) and WRITE_SIZE counters of the walk on this workload
+# (profiles/k_walk_fc_1e12_pmc.md): p, v, one' and both residues read, both
+# residues written.
+WALK_FETCH_PER_CAND = 36.15
+WALK_WRITE_PER_CAND = 16.00
 
 
 def composites_in(nmin, nmax):
@@ -241,11 +247,13 @@ def main():
                    "walk_launches": walk_launches, "walk_ms_per_launch": walk_ms / max(walk_launches, 1),
                    "walk_updates_per_s": walk_updates / (walk_ms * 1e-3)},
         "roofline": {"bound": "valu-int32-mul", "achieved": achieved, "peak": p_mul / 1e12,
-                     "unit": "Tops/s", "frac": achieved / (p_mul / 1e12), "traffic": None,
+                     "unit": "Tops/s", "frac": achieved / (p_mul / 1e12),
+                     "traffic": (WALK_FETCH_PER_CAND + WALK_WRITE_PER_CAND) * cand / max(args.steps, 1),
                      "note": f"walk kernel: {W_MUL} INT32 multiplies per update x updates per launch / "
                              f"average launch duration (HIP events); peak = measured lane-op rate "
                              f"(mad_u64_u32 {rates['mad_u64_u32']:.3e}, mul_lo {rates['mul_lo_u32']:.3e}, "
-                             f"mul_hi {rates['mul_hi_u32']:.3e})"},
+                             f"mul_hi {rates['mul_hi_u32']:.3e}); traffic = HBM bytes per walk launch "
+                             f"(PMC bytes per candidate x candidates per segment)"},
     }
     if args.configs:
         line["other_configs"] = other_configs(dev)

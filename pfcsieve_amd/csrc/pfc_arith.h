@@ -84,7 +84,10 @@ PFC_HD uint64_t div3by2_rem(uint32_t u2, uint32_t u1, uint32_t u0, const Modulus
 // correction: returns x with x == (a*n mod d) or x == (a*n mod d) + d, and
 // sets *fix when the caller must subtract d (x >= d).  The walk kernel uses
 // this form so that the rare correction becomes a wavefront-uniform branch.
-PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t d1, uint32_t d0, uint64_t d,
+// nd1 = 2^32 - d1 (the high half of d, negated): u1 - q1*d1 then becomes the
+// low word of one 32x32+64 multiply-add whose 64-bit addend is <u2,u1> as
+// already computed, so no register pair has to be assembled for it.
+PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t nd1, uint32_t d0, uint64_t d,
                                uint32_t v, bool* fix) {
     const uint64_t lo = (uint64_t)(uint32_t)a * n;
     const uint64_t hi = (uint64_t)(uint32_t)(a >> 32) * n + (lo >> 32);   // <u2,u1>
@@ -96,7 +99,8 @@ PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t d1, uint32_t d0,
     // (v_sub_co / v_subb_co on the device).
     uint32_t borrow, borrow2;
     const uint32_t xlo = subb32(u0, (uint32_t)t, 0u, &borrow);
-    const uint32_t xhi = subb32(u1 - q1 * d1, (uint32_t)(t >> 32), borrow, &borrow2);
+    const uint32_t w = (uint32_t)((uint64_t)q1 * nd1 + hi);   // u1 - q1*d1 (mod 2^32)
+    const uint32_t xhi = subb32(w, (uint32_t)(t >> 32), borrow, &borrow2);
     (void)borrow2;
     uint64_t x = ((uint64_t)xhi << 32) | xlo;
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -118,7 +122,7 @@ PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t d1, uint32_t d0,
 PFC_HD uint64_t mulmod32(uint64_t a, uint32_t n, const Modulus& m) {
     bool fix;
     const uint64_t d = mod_d(m);
-    const uint64_t x = mulmod32_nofix(a, n, m.d1, m.d0, d, m.v, &fix);
+    const uint64_t x = mulmod32_nofix(a, n, 0u - m.d1, m.d0, d, m.v, &fix);
     return fix ? x - d : x;
 }
 

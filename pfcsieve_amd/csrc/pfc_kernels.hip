@@ -214,7 +214,7 @@ struct WalkArgs {
 // steps.
 template <int K>
 struct Lanes {
-    uint32_t d1[K], d0[K], v[K], s[K];
+    uint32_t nd1[K], d0[K], v[K], s[K];   // nd1 = 2^32 - (d >> 32)
     uint64_t d[K], one[K], nmo[K];
     bool live[K];
 };
@@ -224,7 +224,7 @@ __device__ __forceinline__ void lanes_from_state(Lanes<K>& L, uint64_t p, uint32
                                                  int k) {
     L.s[k] = pfc::clz64(p);
     L.d[k] = p << L.s[k];
-    L.d1[k] = (uint32_t)(L.d[k] >> 32);
+    L.nd1[k] = 0u - (uint32_t)(L.d[k] >> 32);
     L.d0[k] = (uint32_t)L.d[k];
     L.v[k] = v;
     L.one[k] = one;
@@ -238,7 +238,7 @@ __device__ __forceinline__ bool step(uint64_t (&r)[K], uint32_t n, const Lanes<K
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         bool f;
-        r[k] = pfc::mulmod32_nofix(r[k], n, L.d1[k], L.d0[k], L.d[k], L.v[k], &f);
+        r[k] = pfc::mulmod32_nofix(r[k], n, L.nd1[k], L.d0[k], L.d[k], L.v[k], &f);
         fix |= f;
     }
     // The second correction of the 3-by-2 division is rare: take it as a
@@ -466,7 +466,7 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
         }
         if (it.flags & kBatchCheck) {
             Modulus m;
-            m.d1 = L.d1[k];
+            m.d1 = 0u - L.nd1[k];
             m.d0 = L.d0[k];
             m.v = L.v[k];
             m.s = L.s[k];
