@@ -54,7 +54,12 @@ def cpu_baseline(modes, p0, width, nmin, nmax, steps_per_cand):
                       f"{dt:.1f} s"}
 
 
-def mix_compare(dev, units, verify_threads=16):
+def _progress(on, msg):
+    if on:
+        print(msg, file=sys.stderr, flush=True)
+
+
+def mix_compare(dev, units, verify_threads=16, progress=False):
     """Plain (one job at a time, one range per launch sequence) against batched
     (all units through one batcher, shared persistent launches) on the same
     seeded unit mix.  Returns updates/s of both and the ratio."""
@@ -71,6 +76,7 @@ def mix_compare(dev, units, verify_threads=16):
         cand_plain.append(st.primecount)
         sig_plain.append((st.dev_checksum, st.primecount, st.raw_factors))
         j.destroy()
+        _progress(progress, f"plain: unit {len(cand_plain)}/{len(units)} done, {time.perf_counter() - t:.1f} s")
     t_plain = time.perf_counter() - t
     # batched
     jobs = [P.Job(dev, *u.desc()) for u in units]
@@ -82,6 +88,7 @@ def mix_compare(dev, units, verify_threads=16):
     while b.pending():
         b.round(0)
         rounds += 1
+        _progress(progress, f"batched: round {rounds} done, {b.pending()} jobs pending, {time.perf_counter() - t:.1f} s")
     t_batch = time.perf_counter() - t
     cand_batch = [j.stats().primecount for j in jobs]
     sig_batch = [(j.stats().dev_checksum, j.stats().primecount, j.stats().raw_factors) for j in jobs]
@@ -170,6 +177,22 @@ def other_configs(dev):
                            "steps_per_candidate": steps, "wall_s": dt,
                            "updates_per_s": s.primecount * steps / dt,
                            "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
+    # Start-residue set-up at large nmin (the dominant cost of the uneven mixes):
+    # -! n [1e6, 1e6+256) from p = 1e15, one p-segment.  Set-up time is the
+    # segment's device time minus its walk launches (generate and check are
+    # ~3 ms of it).
+    nchunks, nbits, mont_k = P.setup_table_size(0, 10**6 - 1)
+    products = nchunks + nbits - 1 + 2 * mont_k.bit_length() + 1
+    j = P.Job(dev, 1, 10**15, 10**15 + 2 * 10**9, 10**6, 10**6 + 256)
+    j.run(1)
+    s = j.stats()
+    j.destroy()
+    setup_s = (s.device_ms - s.walk_ms) * 1e-3
+    out["setup_nmin_1e6"] = {"workload": "-! n [1e6,1e6+256) p from 1e15, one p-segment: start residues "
+                                         "Mont(999999!) per candidate",
+                             "candidates": s.primecount, "montgomery_products_per_candidate": products,
+                             "setup_s": setup_s, "candidates_per_s": s.primecount / setup_s,
+                             "products_per_s": s.primecount * products / setup_s}
     return out
 
 
@@ -191,6 +214,7 @@ def main():
                     help="jobs of the seeded served stream to run through the serving loop (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run one plain copy of this many bytes first (HBM counter calibration)")
+    ap.add_argument("--progress", action="store_true", help="progress lines on stderr during long mixes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding over p is "
@@ -260,7 +284,7 @@ def main():
     if args.mix_units:
         from pfcsieve_amd import workload as W
         units = W.mix(256, seed=12345)[: args.mix_units]
-        line["batched"] = mix_compare(dev, units)
+        line["batched"] = mix_compare(dev, units, progress=args.progress)
         line["batched"]["mix"] = ("first %d of 256 seeded units (splitmix64 seed 12345): modes F/P/C equal odds, "
                                   "n-width log-U[1e3,1e6] from 101+U[0,1e6], p-width log-U[1e6,1e9] from "
                                   "log-U[1e10,2^63]" % args.mix_units)

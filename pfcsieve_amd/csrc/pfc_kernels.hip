@@ -147,6 +147,8 @@ __global__ void k_small_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint32_t
 // The tables are uniform across the wavefront and are read with scalar loads.
 typedef const uint32_t __attribute__((address_space(4)))* ConstU32;
 typedef const uint64_t __attribute__((address_space(4)))* ConstU64;
+typedef uint32_t U32x16 __attribute__((ext_vector_type(16)));
+typedef const U32x16 __attribute__((address_space(4)))* ConstU32x16;
 
 __device__ __forceinline__ uint64_t horner_start(const uint64_t* chunks, const uint32_t* groups,
                                                  uint32_t nbits, uint64_t k, const Modulus& m,
@@ -282,18 +284,37 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
             if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_FACTORIAL, L, W.fac, W.fcap, W.ctrl);
         }
     } else if (MODE == 1) {
-        // The primes are fetched 16 at a time (one scalar block load) so the
-        // wavefront does not wait on a scalar load every step.
+        // The primes are fetched 16 at a time with one 64-byte scalar block
+        // load (s_load_dwordx16) when the list is 64-byte aligned, which
+        // device allocations are: one address computation and one load per
+        // 16 steps keeps the scalar pipe free for the step bookkeeping.
         const ConstU32 list = (ConstU32)W.list;
         uint32_t j = j0;
-        for (; j + 16 <= j1; j += 16) {
-            uint32_t q[16];
-#pragma unroll
-            for (int t = 0; t < 16; ++t) q[t] = list[j + t];
-#pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                const uint32_t n = __builtin_amdgcn_readfirstlane(q[t]);
+        if ((reinterpret_cast<uintptr_t>(W.list) & 63u) == 0) {
+            const uint32_t jal = min(j1, (j0 + 15u) & ~15u);
+            for (; j < jal; ++j) {
+                const uint32_t n = __builtin_amdgcn_readfirstlane(list[j]);
                 if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_PRIMORIAL, L, W.fac, W.fcap, W.ctrl);
+            }
+            const ConstU32x16 blocks = (ConstU32x16)W.list;
+            for (; j + 16 <= j1; j += 16) {
+                const U32x16 q = blocks[j >> 4];
+#pragma unroll
+                for (int t = 0; t < 16; ++t) {
+                    const uint32_t n = __builtin_amdgcn_readfirstlane(q[t]);
+                    if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_PRIMORIAL, L, W.fac, W.fcap, W.ctrl);
+                }
+            }
+        } else {
+            for (; j + 16 <= j1; j += 16) {
+                uint32_t q[16];
+#pragma unroll
+                for (int t = 0; t < 16; ++t) q[t] = list[j + t];
+#pragma unroll
+                for (int t = 0; t < 16; ++t) {
+                    const uint32_t n = __builtin_amdgcn_readfirstlane(q[t]);
+                    if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_PRIMORIAL, L, W.fac, W.fcap, W.ctrl);
+                }
             }
         }
         for (; j < j1; ++j) {

@@ -229,3 +229,45 @@ def test_stages_individually(orc, dev, modes, nmin, nmax):
     c = np.frombuffer(ctrl.download(), np.uint64)
     ref = orc.run(modes, lo, hi, nmin, nmax)
     assert int(c[2]) == ref.dev_checksum and int(c[3]) == ref.primecount == n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("offset", [0, 1])
+def test_primorial_walk_stage(orc, dev, offset):
+    """Primorial walk through the stage ABI with walk boundaries that are not
+    multiples of 16 steps, once with the prime list at a 64-byte aligned
+    device address (16-prime block loads) and once 4 bytes past it (per-prime
+    loads); residues are folded by the check stage and compared with the
+    oracle's checksum."""
+    import pfcsieve_amd._native as Nt
+    from pfcsieve_amd.sieve import DeviceBuffer
+    L = Nt.lib()
+    nmin, nmax = 2000, 2600
+    lo, hi = 10**10, 10**10 + 50000
+    cap = 8192
+    bufs = {k: DeviceBuffer(dev, cap * w) for k, w in [("p", 8), ("v", 4), ("one", 8), ("ra", 8)]}
+    ctrl = DeviceBuffer(dev, 32)
+    ctrl.zero()
+    fac = DeviceBuffer(dev, 16 * 1024)
+    Nt.check(L.pfc_stage_generate(lo, hi, bufs["p"].ptr, cap, ctrl.ptr, None))
+    ta, keep_a = _table(dev, 1, nmin - 1)
+    Nt.check(L.pfc_stage_setup(bufs["p"].ptr, ctrl.ptr, bufs["v"].ptr, bufs["one"].ptr, bufs["ra"].ptr,
+                               None, C.byref(ta), None, cap, None))
+    primes = np.array([int(x) for x in orc.primes(nmin, nmax - 1)], np.uint32)
+    assert len(primes) > 48
+    host = np.concatenate([np.zeros(offset, np.uint32), primes])
+    dlist = DeviceBuffer(dev, host.nbytes)
+    dlist.upload(host.tobytes())
+    base = dlist.ptr.value + 4 * offset
+    assert (base % 64 == 0) == (offset == 0)
+    for j0, j1 in [(0, 37), (37, 38), (38, len(primes))]:
+        Nt.check(L.pfc_stage_walk(P, j0, j1, C.c_void_p(base), None, nmin, bufs["p"].ptr, bufs["v"].ptr,
+                                  bufs["one"].ptr, bufs["ra"].ptr, None, ctrl.ptr, fac.ptr, 1024, cap, None))
+    Nt.check(L.pfc_stage_check(P, nmax, bufs["p"].ptr, bufs["v"].ptr, bufs["one"].ptr, bufs["ra"].ptr,
+                               None, ctrl.ptr, cap, None))
+    Nt.check(L.pfc_device_synchronize(dev.handle))
+    c = np.frombuffer(ctrl.download(), np.uint64)
+    u = np.frombuffer(ctrl.download(), np.uint32)
+    ref = orc.run(P, lo, hi, nmin, nmax)
+    assert int(c[2]) == ref.dev_checksum and int(c[3]) == ref.primecount == int(u[0])
+    assert int(u[1]) == ref.raw_factors
