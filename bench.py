@@ -177,6 +177,17 @@ def other_configs(dev):
                            "steps_per_candidate": steps, "wall_s": dt,
                            "updates_per_s": s.primecount * steps / dt,
                            "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
+    # Single-mode walks of the uneven mixes at the headline shape: -! and -c
+    # alone, n in [101, 1e6), p from 1e12, one p-segment each.
+    for name, modes in (("f_segment", 1), ("c_segment", 4)):
+        steps = W.walk_steps(W.Unit(modes, 0, 0, 101, 10**6))
+        j = P.Job(dev, modes, 10**12, 10**12 + 2 * 10**9, 101, 10**6)
+        j.run(1)
+        s = j.stats()
+        j.destroy()
+        out[name] = {"workload": ("-!" if modes == 1 else "-c") + " n [101,1e6) p from 1e12, one p-segment",
+                     "candidates": s.primecount, "steps_per_candidate": steps,
+                     "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
     # Start-residue set-up at large nmin (the dominant cost of the uneven mixes):
     # -! n [1e6, 1e6+256) from p = 1e15, one p-segment.  Set-up time is the
     # segment's device time minus its walk launches (generate and check are

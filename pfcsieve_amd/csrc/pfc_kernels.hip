@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
 
 // ------------------------------------------------------------------ walk
 
-__device__ __noinline__ void record_factor(pfc_factor* fac, uint32_t fcap, pfc_ctrl* ctrl,
+__device__ __forceinline__ void record_factor(pfc_factor* fac, uint32_t fcap, pfc_ctrl* ctrl,
                                            uint64_t p, uint32_t n, bool plus, int32_t type) {
     const uint32_t idx = atomicAdd(&ctrl->nfactor, 1u);
     if (idx < fcap) {
@@ -233,7 +233,10 @@ __device__ __forceinline__ void lanes_from_state(Lanes<K>& L, uint64_t p, uint32
     L.nmo[k] = L.d[k] - one;
 }
 
-// r[k] <- r[k] * n mod d[k] for all k; returns true if any residue is +-1.
+// r[k] <- r[k] * n mod d[k] for all k; returns true, uniformly across the
+// wavefront, if any residue of any lane is +-1.  The two compares are
+// balloted separately so that the no-hit case costs one scalar OR and one
+// branch (no exec-mask save/restore); the record path re-tests each lane.
 template <int K>
 __device__ __forceinline__ bool step(uint64_t (&r)[K], uint32_t n, const Lanes<K>& L) {
     bool fix = false;
@@ -250,10 +253,19 @@ __device__ __forceinline__ bool step(uint64_t (&r)[K], uint32_t n, const Lanes<K
         for (int k = 0; k < K; ++k)
             if (r[k] >= L.d[k]) r[k] -= L.d[k];
     }
-    bool hit = false;
+    uint64_t hits = 0;
 #pragma unroll
-    for (int k = 0; k < K; ++k) hit |= (r[k] == L.one[k]) | (r[k] == L.nmo[k]);
-    return hit;
+    for (int k = 0; k < K; ++k) hits |= __ballot(r[k] == L.one[k]) | __ballot(r[k] == L.nmo[k]);
+    return __builtin_expect(hits != 0, 0);
+}
+
+// Per-lane re-test and record, out of line: the walk reaches it only through
+// the wavefront-uniform hit branch, so the compiler cannot fold the per-lane
+// condition back into an exec-masked region on the hot path.
+__device__ __noinline__ void record_lane(uint64_t r, uint64_t one, uint64_t nmo, uint64_t d, uint32_t s,
+                                         bool live, uint32_t n, int32_t type, pfc_factor* fac,
+                                         uint32_t fcap, pfc_ctrl* ctrl) {
+    if (live && (r == one || r == nmo)) record_factor(fac, fcap, ctrl, d >> s, n, r == nmo, type);
 }
 
 template <int K>
@@ -262,8 +274,7 @@ __device__ __forceinline__ void record_hits(const uint64_t (&r)[K], uint32_t n, 
                                             pfc_ctrl* ctrl) {
 #pragma unroll
     for (int k = 0; k < K; ++k)
-        if (L.live[k] && (r[k] == L.one[k] || r[k] == L.nmo[k]))
-            record_factor(fac, fcap, ctrl, L.d[k] >> L.s[k], n, r[k] == L.nmo[k], type);
+        record_lane(r[k], L.one[k], L.nmo[k], L.d[k], L.s[k], L.live[k], n, type, fac, fcap, ctrl);
 }
 
 struct WalkStream {
@@ -279,7 +290,16 @@ template <int MODE, int K>
 __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K], const Lanes<K>& L,
                                            uint32_t j0, uint32_t j1, const WalkStream& W) {
     if (MODE == 0) {
-        for (uint32_t j = j0; j < j1; ++j) {
+        // Four steps per trip: one scalar compare-and-branch per four updates.
+        uint32_t j = j0;
+        for (; j + 4 <= j1; j += 4) {
+#pragma unroll
+            for (uint32_t t = 0; t < 4; ++t) {
+                const uint32_t n = __builtin_amdgcn_readfirstlane(j + t);
+                if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_FACTORIAL, L, W.fac, W.fcap, W.ctrl);
+            }
+        }
+        for (; j < j1; ++j) {
             const uint32_t n = __builtin_amdgcn_readfirstlane(j);
             if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_FACTORIAL, L, W.fac, W.fcap, W.ctrl);
         }
@@ -322,13 +342,34 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
             if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_PRIMORIAL, L, W.fac, W.fcap, W.ctrl);
         }
     } else {
+        // F|C (MODE 2) and C (MODE 3): composite flags come from the bitmap,
+        // one 32-bit word per 32 steps.  A whole word inside [j0, j1) takes an
+        // unrolled path whose flag tests use constant bit indices (one scalar
+        // bit-compare and branch per n, no shifts or loop counter); partial
+        // words at either end of the range take the generic loop.
         const ConstU32 comp = (ConstU32)W.composite;
         uint32_t j = j0;
         while (j < j1) {
             const uint32_t off = j - W.nbase;
             const uint32_t wbase = off & ~31u;
-            const uint32_t jend = min(j1, W.nbase + wbase + 32u);
+            const uint32_t nword = W.nbase + wbase;
             const uint32_t word = __builtin_amdgcn_readfirstlane(comp[wbase >> 5]);
+            if ((off & 31u) == 0u && j1 - j >= 32u) {
+#pragma unroll
+                for (uint32_t t = 0; t < 32; ++t) {
+                    const uint32_t n = __builtin_amdgcn_readfirstlane(nword + t);
+                    if (MODE == 2) {
+                        if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_FACTORIAL, L, W.fac, W.fcap, W.ctrl);
+                        if (((word >> t) & 1u) && step<K>(rb, n, L))
+                            record_hits<K>(rb, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
+                    } else if (((word >> t) & 1u) && step<K>(ra, n, L)) {
+                        record_hits<K>(ra, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
+                    }
+                }
+                j = nword + 32u;
+                continue;
+            }
+            const uint32_t jend = min(j1, nword + 32u);
             for (; j < jend; ++j) {
                 const uint32_t n = __builtin_amdgcn_readfirstlane(j);
                 const bool composite = (word >> ((n - W.nbase) & 31u)) & 1u;
