@@ -42,16 +42,48 @@ def composites_in(nmin, nmax):
     return int((~sieve[nmin:nmax]).sum())
 
 
-def cpu_baseline(modes, p0, width, nmin, nmax, steps_per_cand):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(modes, p0, width, nmin, nmax, steps_per_cand, width3=600_000):
+    """The oracle (C, OpenMP over candidates, all host threads) on a bounded
+    p-sample of the headline configuration, plus the reference's CPU-runnable
+    case in full and a p-sample of the primorial configuration."""
     import oracle as O
+    from pfcsieve_amd import workload as W
     O.build()
     t = time.perf_counter()
     r = O.run(modes, p0, p0 + width, nmin, nmax)
     dt = time.perf_counter() - t
     upd = r.primecount * steps_per_cand
-    return {"value": upd / dt, "unit": "updates/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"-! -c p in [{p0}, {p0 + width}) ({r.primecount} candidates), n in [{nmin}, {nmax}); "
-                      f"{dt:.1f} s"}
+    out = {"value": upd / dt, "unit": "updates/s", "cores": O.num_threads(), "kind": "port",
+           "sample": f"-! -c p in [{p0}, {p0 + width}) ({r.primecount} candidates), n in [{nmin}, {nmax}); "
+                     f"{dt:.1f} s",
+           "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
+    # cfg1 in full: -! -n 101 -N 5000 -p 1e9 -P 1.001e9
+    t = time.perf_counter()
+    r1 = O.run(1, 10**9, 10**9 + 10**6, 101, 5000)
+    dt1 = time.perf_counter() - t
+    out["cfg1_full"] = {"candidates": r1.primecount, "s": dt1, "updates_per_s": r1.primecount * 4899 / dt1,
+                        "checksum": "%016X" % r1.checksum}
+    # cfg3 p-sample: -# -n 101 -N 1e7 from p = 2^62 - 4e9
+    p3 = 2**62 - 4 * 10**9
+    steps3 = W.walk_steps(W.Unit(2, 0, 0, 101, 10**7))
+    t = time.perf_counter()
+    r3 = O.run(2, p3, p3 + width3, 101, 10**7)
+    dt3 = time.perf_counter() - t
+    out["cfg3_sample"] = {"sample": f"-# p in [{p3}, {p3 + width3}), n in [101, 1e7)",
+                          "candidates": r3.primecount, "s": dt3,
+                          "updates_per_s": r3.primecount * steps3 / dt3}
+    return out
 
 
 def _progress(on, msg):
@@ -216,7 +248,7 @@ def main():
     ap.add_argument("--ppl", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-width", type=int, default=150000)
+    ap.add_argument("--cpu-width", type=int, default=1_000_000)
     ap.add_argument("--mix-units", type=int, default=24,
                     help="units of the seeded 256-unit batched mix to compare plain vs batched (0 = skip)")
     ap.add_argument("--autotune", action="store_true")

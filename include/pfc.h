@@ -74,6 +74,8 @@ typedef struct pfc_tuning {
     uint32_t seg_candidates;   /* target candidates per p-segment */
     uint32_t use_graph;        /* 1: capture a job's segment pipeline (clear, generate, setup,
                                   walk chunks, check) once as a HIP graph and replay it per segment */
+    uint32_t skip_table_check; /* 1: do not self-check the start tables at job creation
+                                  (pfc_stage_verify_table); 0 (default): check them */
 } pfc_tuning;
 
 /* Host-side summary of a finished job (cl_sieve.cpp:1748). */
@@ -249,6 +251,22 @@ int pfc_stage_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const ui
 int pfc_build_setup_table(uint32_t kind, uint32_t N, uint64_t* chunks, size_t chunk_cap,
                           size_t* nchunks, uint32_t* groups, size_t group_cap, uint32_t* nbits,
                           uint64_t* mont_k);
+
+/* Start-table self-check (verifyslow.cl, verify.cl, verifyresult.cl;
+ * cl_sieve.cpp:917-970, 1079-1134, 1246-1304): X(N) of a table of `kind`
+ * (0: N!, 1: N#, 2: N!/N#) modulo the test prime 2^64-59, once as the
+ * product of every multiplier up to N (primality of each n decided on the
+ * device by a deterministic 32-bit Miller-Rabin) and once through the
+ * table's start-residue path.  d_work: device buffer of PFC_VERIFY_WORDS u64;
+ * afterwards d_work[0] = Mont(X) from the product and d_work[1] = the
+ * table's start residue; the table is right when they are equal.
+ * pfc_job_create runs this for every table it builds unless
+ * pfc_tuning.skip_table_check is set, and fails with PFC_ERR_VALIDATION
+ * ("power table verification failed") on a mismatch. */
+#define PFC_VERIFY_BLOCKS 256
+#define PFC_VERIFY_WORDS (2 + PFC_VERIFY_BLOCKS)
+int pfc_stage_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_work,
+                           void* stream);
 
 /* ------------------------------------------------------------------ probe */
 /* Measures the lane-op rate of independent v_mad_u64_u32 / v_mul_lo_u32 /

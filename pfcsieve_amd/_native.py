@@ -34,9 +34,13 @@ class JobDesc(C.Structure):
                 ("pmin", C.c_uint64), ("pmax", C.c_uint64), ("factor_capacity", C.c_uint64)]
 
 
+VERIFY_WORDS = 2 + 256  # PFC_VERIFY_WORDS
+
+
 class Tuning(C.Structure):
     _fields_ = [("primes_per_lane", C.c_uint32), ("block_size", C.c_uint32),
-                ("walk_chunk", C.c_uint32), ("seg_candidates", C.c_uint32), ("use_graph", C.c_uint32)]
+                ("walk_chunk", C.c_uint32), ("seg_candidates", C.c_uint32), ("use_graph", C.c_uint32),
+                ("skip_table_check", C.c_uint32)]
 
 
 class JobStats(C.Structure):
@@ -94,6 +98,7 @@ SIGNATURES = {
     "pfc_stage_walk": (_int, [_u32, _u32, _u32, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                               _u32, _u32, _vp]),
     "pfc_stage_check": (_int, [_u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
+    "pfc_stage_verify_table": (_int, [_u32, _u32, C.POINTER(SetupTable), _vp, _vp]),
     "pfc_build_setup_table": (_int, [_u32, _u32, C.POINTER(_u64), _sz, C.POINTER(_sz),
                                      C.POINTER(_u32), _sz, C.POINTER(_u32), C.POINTER(_u64)]),
     "pfc_probe_mul_rates": (_int, [_vp, C.POINTER(C.c_double * 3)]),

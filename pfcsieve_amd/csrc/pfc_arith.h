@@ -225,6 +225,60 @@ PFC_HD bool strong_prp2(uint64_t N) {
     return false;
 }
 
+// ---------------------------------------------------------------- 32-bit primality
+
+// Montgomery product a*b*2^-32 mod n for odd n < 2^32 and a, b < n;
+// nneg = -n^-1 mod 2^32.
+PFC_HD uint32_t mont32_mul(uint32_t a, uint32_t b, uint32_t n, uint32_t nneg) {
+    const uint64_t t = (uint64_t)a * b;
+    const uint32_t q = (uint32_t)t * nneg;
+    const uint64_t qn = (uint64_t)q * n;
+    // t + qn == 0 mod 2^32; the low words add to 0 or 2^32.
+    const uint64_t carry = ((uint64_t)(uint32_t)t + (uint32_t)qn) >> 32;
+    const uint64_t r = (t >> 32) + (qn >> 32) + carry;   // < 2n
+    return (uint32_t)(r >= n ? r - n : r);
+}
+
+// Deterministic primality of a 32-bit integer: strong probable-prime tests to
+// the bases 2, 7 and 61 (Jaeschke 1993: the least composite passing all
+// three is 4,759,123,141 > 2^32).  Used by the start-table self-check as a
+// primality source independent of the host sieve.
+PFC_HD bool is_prime_u32(uint32_t n) {
+    if (n < 2) return false;
+    if (n < 4) return true;
+    if (!(n & 1u)) return false;
+    uint32_t inv = n;                          // n*n == 1 mod 8: 3 correct bits
+    for (int i = 0; i < 4; ++i) inv *= 2u - n * inv;
+    const uint32_t nneg = 0u - inv;
+    const uint32_t one = (uint32_t)(0x100000000ull % n);   // 2^32 mod n
+    const uint32_t mone = n - one;                          // -1
+    uint32_t d = n - 1;
+    uint32_t t = 0;
+    while (!(d & 1u)) {
+        d >>= 1;
+        ++t;
+    }
+    const uint32_t bases[3] = {2u, 7u, 61u};
+    for (int b = 0; b < 3; ++b) {
+        const uint32_t a = bases[b] % n;
+        if (a == 0) continue;
+        const uint32_t am = (uint32_t)(((uint64_t)a << 32) % n);
+        uint32_t x = one;
+        for (int bit = 31 - __builtin_clz(d); bit >= 0; --bit) {
+            x = mont32_mul(x, x, n, nneg);
+            if ((d >> bit) & 1u) x = mont32_mul(x, am, n, nneg);
+        }
+        if (x == one || x == mone) continue;
+        bool witness = true;
+        for (uint32_t i = 1; i < t && witness; ++i) {
+            x = mont32_mul(x, x, n, nneg);
+            if (x == mone) witness = false;
+        }
+        if (witness) return false;
+    }
+    return true;
+}
+
 // ---------------------------------------------------------------- Montgomery (start residues)
 
 // p^-1 mod 2^64 for odd p (Newton: each step doubles the correct low bits;

@@ -76,10 +76,13 @@ struct HostTable {
     std::vector<uint32_t> groups;  // nbits + 1 offsets
     uint32_t nbits = 0;
     uint64_t mont_k = 1;           // 1 - E (see pfc_build_setup_table)
+    uint32_t kind = 0, N = 0;      // X(N) = N!, N# or N!/N#
 };
 
 HostTable build_table(uint32_t kind, uint32_t N) {
     HostTable T;
+    T.kind = kind;
+    T.N = N;
     std::vector<uint32_t> pr = primes_in(2, N);
     std::vector<uint32_t> ex(pr.size());
     uint32_t maxe = 0;
@@ -151,6 +154,7 @@ static pfc_tuning default_tuning() {
     t.walk_chunk = 1u << 16;
     t.seg_candidates = 1u << 22;
     t.use_graph = 0;
+    t.skip_table_check = 0;
     return t;
 }
 
@@ -427,6 +431,34 @@ uint64_t footprint_of(const pfc_job_desc* d, const pfc_tuning& t) {
     return bytes;
 }
 
+// Device self-check of the job's start tables against the product of their
+// multipliers modulo 2^64-59 (cl_sieve.cpp:917-970, 1079-1134, 1246-1304).
+int verify_tables(pfc_job* j) {
+    const bool two = (j->desc.modes & PFC_MODE_FACTORIAL) && (j->desc.modes & PFC_MODE_COMPOSITORIAL);
+    uint64_t* d_work = nullptr;
+    hipError_t e = hipMalloc((void**)&d_work, PFC_VERIFY_WORDS * sizeof(uint64_t));
+    if (e != hipSuccess) return set_error(PFC_ERR_NOMEM, "table check: device allocation failed");
+    const HostTable* hs[2] = {&j->ta, &j->tb};
+    const uint64_t* cs[2] = {j->d_ca, j->d_cb};
+    const uint32_t* gs[2] = {j->d_ga, j->d_gb};
+    int rc = PFC_OK;
+    for (int i = 0; i < (two ? 2 : 1) && !rc; ++i) {
+        const pfc_setup_table t{cs[i], gs[i], hs[i]->nbits, hs[i]->mont_k};
+        rc = pfc::launch_verify_table(hs[i]->kind, hs[i]->N, &t, d_work, j->stream);
+        uint64_t got[2] = {0, 1};
+        if (!rc) {
+            e = hipMemcpyAsync(got, d_work, sizeof got, hipMemcpyDeviceToHost, j->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(j->stream);
+            if (e != hipSuccess) rc = hip_fail(e, "table check");
+        }
+        if (!rc && got[0] != got[1])
+            rc = set_error(PFC_ERR_VALIDATION, hs[i]->kind == 0 ? "error: power table verification failed"
+                                                                : "error: product/prime table verification failed");
+    }
+    hipFree(d_work);
+    return rc;
+}
+
 }  // namespace
 
 extern "C" int pfc_job_footprint(const pfc_device* dev, const pfc_job_desc* desc, uint64_t* bytes) {
@@ -499,6 +531,7 @@ extern "C" int pfc_job_create(pfc_device* dev, const pfc_job_desc* desc, pfc_job
         e = hipStreamSynchronize(j->stream);
         if (e != hipSuccess) rc = hip_fail(e, "table upload");
     }
+    if (!rc && !j->tuning.skip_table_check) rc = verify_tables(j);
     if (rc) {
         free_job_device(j);
         delete j;

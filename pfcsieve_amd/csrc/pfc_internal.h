@@ -58,6 +58,8 @@ int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint3
                  pfc_ctrl* d_ctrl, uint32_t cap, hipStream_t st);
 int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_item* d_items, uint32_t nitems,
                  uint32_t* d_head, uint32_t ppl, uint32_t nblocks, hipStream_t st);
+int launch_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_work,
+                        hipStream_t st);
 int probe_mul_rates(int cus, double rates[3]);
 int probe_copy(uint64_t bytes, double* ms);
 

@@ -178,6 +178,49 @@ __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
     }
 }
 
+// ------------------------------------------------------------------ table self-check
+
+// X(N) mod the test prime 2^64 - 59 two ways (the reference's verifyslow.cl /
+// verify.cl / verifyresult.cl pair, cl_sieve.cpp:917-970, 1079-1134,
+// 1246-1304): the product of every multiplier up to N with the walk's
+// 32-bit modular product (k_verify_slow: n for N!, primes for N#, composites
+// for N!/N#, primality from is_prime_u32, not from the host sieve that built
+// the table), and the table's start residue through the set-up code path
+// (k_verify_finish).  work[0] = Mont(X) from the product, work[1] = the
+// table's start residue, work[2 .. 2 + kVerifyBlocks) = block partials.
+constexpr uint64_t kTestPrime = 18446744073709551557ull;  // 2^64 - 59 (verifyslow.cl:46)
+
+__global__ __launch_bounds__(256) void k_verify_slow(uint32_t kind, uint32_t N, uint64_t* __restrict__ work) {
+    const Modulus m = pfc::make_modulus(kTestPrime);     // s = 0: residues are plain
+    uint64_t acc = 1;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t n = 2 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; n <= N; n += stride) {
+        const uint32_t q = (uint32_t)n;
+        if (kind != 0 && (kind == 1) != pfc::is_prime_u32(q)) continue;
+        acc = pfc::mulmod32(acc, q, m);
+    }
+    __shared__ uint64_t sh[256];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sh[threadIdx.x] = pfc::mulmod64(sh[threadIdx.x], sh[threadIdx.x + h], m);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) work[2 + blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(64) void k_verify_finish(const uint64_t* __restrict__ chunks,
+                                                      const uint32_t* __restrict__ groups, uint32_t nbits,
+                                                      uint64_t k, uint64_t* __restrict__ work) {
+    if (threadIdx.x != 0) return;
+    const Modulus m = pfc::make_modulus(kTestPrime);
+    uint64_t x = 1;
+    for (uint32_t i = 0; i < PFC_VERIFY_BLOCKS; ++i) x = pfc::mulmod64(x, work[2 + i], m);
+    const uint64_t one = pfc::mont_one(m);
+    work[0] = pfc::mulmod64(one, x, m);                  // X * R mod p
+    work[1] = horner_start(chunks, groups, nbits, k, m, one);
+}
+
 // ------------------------------------------------------------------ walk
 
 __device__ __forceinline__ void record_factor(pfc_factor* fac, uint32_t fcap, pfc_ctrl* ctrl,
@@ -734,6 +777,17 @@ int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_item* d_items, uin
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_batch launch");
 }
 
+int launch_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_work,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_verify_slow, dim3(PFC_VERIFY_BLOCKS), dim3(256), 0, st, kind, N, d_work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "k_verify_slow launch");
+    hipLaunchKernelGGL(k_verify_finish, dim3(1), dim3(64), 0, st, t->chunks, t->groups, t->nbits, t->mont_k,
+                       d_work);
+    e = hipGetLastError();
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "k_verify_finish launch");
+}
+
 int probe_copy(uint64_t bytes, double* ms_out) {
     const uint64_t n = bytes / 16;
     if (n == 0) return set_error(PFC_ERR_INVALID, "probe_copy: size below 16 bytes");
@@ -832,6 +886,17 @@ int pfc_stage_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const ui
         return pfc::set_error(PFC_ERR_INVALID, "check: res_b missing");
     return pfc::launch_check(modes, nmax, d_p, d_v, d_one, d_res_a, d_res_b, d_ctrl, cap,
                              (hipStream_t)stream);
+}
+
+int pfc_stage_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_work,
+                           void* stream) {
+    pfc::g_err[0] = 0;
+    if (!t || !d_work) return pfc::set_error(PFC_ERR_INVALID, "verify_table: null pointer");
+    if (kind > 2) return pfc::set_error(PFC_ERR_INVALID, "verify_table: kind must be 0, 1 or 2");
+    if (N > 0x7FFFFFFFu) return pfc::set_error(PFC_ERR_INVALID, "verify_table: N >= 2^31");
+    if (t->nbits && (!t->chunks || !t->groups))
+        return pfc::set_error(PFC_ERR_INVALID, "verify_table: table pointers missing");
+    return pfc::launch_verify_table(kind, N, t, d_work, (hipStream_t)stream);
 }
 
 }  // extern "C"

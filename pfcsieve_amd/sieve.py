@@ -38,8 +38,9 @@ class Device:
         check(lib().pfc_device_info(self._h, name, 128, C.byref(cus), C.byref(hbm)))
         return {"name": name.value.decode(), "compute_units": cus.value, "hbm_bytes": hbm.value}
 
-    def set_tuning(self, primes_per_lane=0, block_size=0, walk_chunk=0, seg_candidates=0, use_graph=0):
-        t = Tuning(primes_per_lane, block_size, walk_chunk, seg_candidates, use_graph)
+    def set_tuning(self, primes_per_lane=0, block_size=0, walk_chunk=0, seg_candidates=0, use_graph=0,
+                   skip_table_check=0):
+        t = Tuning(primes_per_lane, block_size, walk_chunk, seg_candidates, use_graph, skip_table_check)
         check(lib().pfc_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:

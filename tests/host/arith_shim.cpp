@@ -19,6 +19,7 @@ uint64_t t_sqrmod(uint64_t p, uint64_t a_shifted) { return pfc::sqrmod(a_shifted
 uint64_t t_mont_one(uint64_t p) { return pfc::mont_one(pfc::make_modulus(p)); }
 int t_modulus_ok(uint64_t p) { return pfc::modulus_ok(p, pfc::make_modulus(p)); }
 int t_strong_prp2(uint64_t n) { return pfc::strong_prp2(n); }
+int t_is_prime_u32(uint32_t n) { return pfc::is_prime_u32(n); }
 uint64_t t_start_residue(const uint64_t* chunks, const uint32_t* groups, uint32_t nbits, uint64_t k,
                          uint64_t p) {
     const pfc::Modulus m = pfc::make_modulus(p);

@@ -161,8 +161,9 @@ def test_graph_replay_matches_eager(orc, dev, job):
     assert (got.checksum, got.primecount, got.factorcount) == (ref.checksum, ref.primecount, ref.factorcount)
 
 
-def _table(dev, kind, N):
-    """pfc_build_setup_table on the host, uploaded to device buffers."""
+def _table(dev, kind, N, corrupt=False):
+    """pfc_build_setup_table on the host, uploaded to device buffers
+    (corrupt: the first chunk is changed before the upload)."""
     import pfcsieve_amd._native as Nt
     from pfcsieve_amd.sieve import DeviceBuffer
     L = Nt.lib()
@@ -172,6 +173,9 @@ def _table(dev, kind, N):
     gr = (C.c_uint32 * (nb.value + 1))()
     Nt.check(L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb),
                                      C.byref(mk)))
+    if corrupt:
+        assert nch.value > 0
+        ch[0] = ch[0] + 2 if ch[0] < 2**63 else ch[0] - 2
     dc = DeviceBuffer(dev, 8 * max(nch.value, 1))
     dc.upload(bytes(ch))
     dg = DeviceBuffer(dev, 4 * (nb.value + 1))
@@ -271,3 +275,59 @@ def test_primorial_walk_stage(orc, dev, offset):
     ref = orc.run(P, lo, hi, nmin, nmax)
     assert int(c[2]) == ref.dev_checksum and int(c[3]) == ref.primecount == int(u[0])
     assert int(u[1]) == ref.raw_factors
+
+
+def _x_of(orc, kind, N, p):
+    """X(N) mod p from its definition: N!, N# or N!/N#."""
+    primes = set(int(q) for q in orc.primes(2, N)) if N >= 2 else set()
+    x = 1
+    for n in range(2, N + 1):
+        if kind == 0 or (kind == 1) == (n in primes):
+            x = x * n % p
+    return x
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("N", [1, 100, 5000, 123457])
+def test_table_self_check(orc, dev, kind, N):
+    """The device self-check of a start table: both sides equal Mont(X(N))
+    modulo 2^64-59 computed from the definition, and a table with one
+    changed chunk is caught."""
+    import pfcsieve_amd._native as Nt
+    from pfcsieve_amd.sieve import DeviceBuffer
+    L = Nt.lib()
+    pt = 2**64 - 59
+    want = _x_of(orc, kind, N, pt) * 2**64 % pt
+    work = DeviceBuffer(dev, 8 * Nt.VERIFY_WORDS)
+    t, keep = _table(dev, kind, N)
+    Nt.check(L.pfc_stage_verify_table(kind, N, C.byref(t), work.ptr, None))
+    Nt.check(L.pfc_device_synchronize(dev.handle))
+    got = np.frombuffer(work.download(16), np.uint64)
+    assert int(got[0]) == want and int(got[1]) == want
+    if t.nbits:
+        tb, keep_b = _table(dev, kind, N, corrupt=True)
+        Nt.check(L.pfc_stage_verify_table(kind, N, C.byref(tb), work.ptr, None))
+        Nt.check(L.pfc_device_synchronize(dev.handle))
+        bad = np.frombuffer(work.download(16), np.uint64)
+        assert int(bad[0]) == want and int(bad[1]) != want
+
+
+@pytest.mark.gpu
+def test_job_create_runs_table_check(dev):
+    """Jobs of every mode are created with the self-check on (the default) and
+    with it skipped, and give the same results."""
+    import pfcsieve_amd as PK
+    res = []
+    for skip in (0, 1):
+        dev.set_tuning(skip_table_check=skip)
+        try:
+            for modes, nmax in ((F, 3000), (P, 3000), (Cm, 3000), (F | Cm, 3000)):
+                j = PK.Job(dev, modes, 10**9, 10**9 + 20000, 2500, nmax)
+                j.run()
+                st = j.stats()
+                res.append((skip, modes, st.dev_checksum, st.primecount))
+                j.destroy()
+        finally:
+            dev.set_tuning()
+    assert [r[1:] for r in res[:4]] == [r[1:] for r in res[4:]]

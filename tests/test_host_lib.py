@@ -158,6 +158,7 @@ def A():
         getattr(L, n).restype = u64
     L.t_modulus_ok.argtypes = [u64]
     L.t_strong_prp2.argtypes = [u64]
+    L.t_is_prime_u32.argtypes = [u32]
     L.t_generate.argtypes = [u64, u64, C.POINTER(u64), C.c_size_t]
     L.t_generate.restype = C.c_size_t
     L.t_start_residue.argtypes = [C.POINTER(u64), C.POINTER(u32), u32, u64, u64]
@@ -205,6 +206,32 @@ def test_strong_prp2_device_arith(A, orc):
     xs += [(1 << t) + 1 for t in (8, 16, 32, 40, 48, 56)] + [3215031751, 2152302898747, 3474749660383]
     for x in xs:
         assert bool(A.t_strong_prp2(x)) == orc.strong_prp2(x), x
+
+
+def test_is_prime_u32(A, orc):
+    """The table self-check's 32-bit primality test: a sieve below 2e5, the
+    strong pseudoprimes to small base sets (all composite), 2^32 edges, and
+    the oracle's 7-base isPrime on random 32-bit integers."""
+    import numpy as np
+    lim = 200_000
+    sv = np.ones(lim, dtype=bool)
+    sv[:2] = False
+    for i in range(2, int(lim ** 0.5) + 1):
+        if sv[i]:
+            sv[i * i::i] = False
+    for n in range(lim):
+        assert bool(A.t_is_prime_u32(n)) == bool(sv[n]), n
+    # composites that pass base 2 (2047...), bases 2,3 (1373653), 2,3,5 (25326001),
+    # 2,3,5,7 (3215031751), and Carmichael numbers
+    for n in [2047, 3277, 4033, 4681, 8321, 1373653, 25326001, 3215031751, 561, 41041, 825265,
+              321197185, 4294967295]:
+        assert not A.t_is_prime_u32(n), n
+    for n in [4294967291, 4294967279, 2147483647, 65537, 61, 7]:
+        assert A.t_is_prime_u32(n), n
+    rng = random.Random(11)
+    for _ in range(20000):
+        n = rng.randrange(1 << 32)
+        assert bool(A.t_is_prime_u32(n)) == orc.is_prime(n), n
 
 
 @pytest.mark.parametrize("lo,hi", [(114, 20000), (119, 120), (10**9, 10**9 + 2 * 10**5),
