@@ -135,7 +135,7 @@ def mix_compare(dev, units, verify_threads=16, progress=False):
             "rounds": rounds, "results_identical": sig_plain == sig_batch}
 
 
-def serve_run(dev, units, max_steps=50_000):
+def serve_run(dev, units, max_steps=50_000, progress=False):
     """Served stream: the units arrive as a seeded Poisson process at 80% of
     the batch capacity measured first on the same units; admission by HBM
     footprint; reports throughput and p50/p99 job latency."""
@@ -144,12 +144,14 @@ def serve_run(dev, units, max_steps=50_000):
     from pfcsieve_amd import workload as W
     steps = [W.walk_steps(u) for u in units]
     t = time.perf_counter()
-    res = P.run_batch([u.desc() for u in units], device=dev, verify_threads=16)
+    say = (lambda m: _progress(True, m)) if progress else None
+    res = P.run_batch([u.desc() for u in units], device=dev, verify_threads=16, progress=say)
     cap_s = time.perf_counter() - t
     rate = 0.8 * len(units) / cap_s
     arrivals = W.poisson_arrivals(len(units), rate, seed=777)
     info = dev.info()
-    srv = S.Server(dev, hbm_budget=int(info["hbm_bytes"] * 0.8), max_steps=max_steps, verify_threads=16)
+    srv = S.Server(dev, hbm_budget=int(info["hbm_bytes"] * 0.8), max_steps=max_steps, verify_threads=16,
+                   progress=say)
     for k, (u, a) in enumerate(zip(units, arrivals)):
         srv.submit(S.Request(k, u.desc(), a))
     st = srv.run()
@@ -334,7 +336,7 @@ def main():
     if args.serve_jobs:
         from pfcsieve_amd import workload as W
         units = W.mix(10_000, seed=2024)[: args.serve_jobs]
-        line["served"] = serve_run(dev, units)
+        line["served"] = serve_run(dev, units, progress=args.progress)
         line["served"]["stream"] = ("first %d of 10,000 seeded jobs (splitmix64 seed 2024, same unit distribution "
                                     "as the batched mix), Poisson arrivals at 0.8x measured batch capacity"
                                     % args.serve_jobs)

@@ -46,7 +46,9 @@ class ServeStats:
 
 
 class Server:
-    def __init__(self, dev: Device, hbm_budget: int, max_steps: int = 20000, verify_threads: int = 2):
+    def __init__(self, dev: Device, hbm_budget: int, max_steps: int = 20000, verify_threads: int = 2,
+                 progress=None):
+        self.progress = progress    # optional callable receiving one line per finished job
         self.dev = dev
         self.budget = hbm_budget
         self.max_steps = max_steps
@@ -104,6 +106,9 @@ class Server:
                 self.done.append(r)
                 st.latencies.append(r.finished - r.arrival)
                 st.jobs += 1
+                if self.progress:
+                    self.progress(f"served job {r.rid} ({st.jobs} done) at {now:.1f} s, "
+                                  f"latency {r.finished - r.arrival:.2f} s")
         st.wall_s = self._now()
         self.batcher.destroy()
         return st

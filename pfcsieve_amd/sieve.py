@@ -282,16 +282,21 @@ def job_result(job: Job, verify_threads: int = 2) -> SieveResult:
 
 
 def run_batch(descs, *, device: Optional[Device] = None, max_steps: int = 0,
-              verify_threads: int = 2) -> List[SieveResult]:
-    """Runs job descriptions (modes, pmin, pmax, nmin, nmax) through one batcher."""
+              verify_threads: int = 2, progress=None) -> List[SieveResult]:
+    """Runs job descriptions (modes, pmin, pmax, nmin, nmax) through one batcher.
+    progress: optional callable receiving one line per round."""
     dev = device or Device(0)
     jobs = [Job(dev, *d) for d in descs]
     b = Batcher(dev)
     try:
         for j in jobs:
             b.add(j)
+        rounds = 0
         while b.pending():
             b.round(max_steps)
+            rounds += 1
+            if progress:
+                progress(f"batch round {rounds}: {b.pending()} of {len(jobs)} jobs pending")
         return [job_result(j, verify_threads) for j in jobs]
     finally:
         b.destroy()
