@@ -106,9 +106,11 @@ int pfc_device_info(const pfc_device* dev, char* name, size_t name_cap, int* com
 int pfc_set_tuning(pfc_device* dev, const pfc_tuning* t);
 int pfc_get_tuning(const pfc_device* dev, pfc_tuning* t);
 /* Project autotuner: times the walk kernel at every primes_per_lane x
- * block_size shape on a sample of `desc` and stores the fastest in the
- * device tuning (also returned in *best).  Counterpart of the reference's
- * profile-driven sizing (cl_sieve.cpp:669-769, 1642-1694). */
+ * block_size shape on a sample of `desc`, then the n-chunk length
+ * (walk_chunk 2^14, 2^16, 2^18 steps per launch) at the fastest shape, and
+ * stores the winners in the device tuning (also returned in *best).
+ * Counterpart of the reference's profile-driven sizing (cl_sieve.cpp:669-769,
+ * 1642-1694). */
 int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tuning* best);
 
 /* ------------------------------------------------------------------ jobs */

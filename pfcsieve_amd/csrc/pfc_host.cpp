@@ -818,6 +818,30 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
             }
         }
     }
+    // n-chunk length (multiplier steps per walk launch) at the chosen shape:
+    // the same step total walked in launches of each length.
+    const uint32_t total = std::min<uint32_t>(1u << 18, j->jend - j->jbegin);
+    const uint32_t chunks[3] = {1u << 14, 1u << 16, 1u << 18};
+    float best_chunk_ms = 1e30f;
+    for (int rep = 0; rep < 2 && !rc; ++rep) {
+        for (uint32_t c : chunks) {
+            if (rc || (c > total && c != chunks[0])) break;
+            (void)hipEventRecord(j->ev0, st);
+            for (uint32_t a = 0; a < total && !rc; a += c)
+                rc = pfc::launch_walk(d.modes, j->jbegin + a, j->jbegin + std::min(total, a + c), j->d_list,
+                                      j->d_comp, d.nmin, j->d_p, j->d_v, j->d_one, j->d_ra, j->d_rb, j->d_ctrl,
+                                      j->d_fac, (uint32_t)j->fcap, cap, win.primes_per_lane, win.block_size, st);
+            (void)hipEventRecord(j->ev1, st);
+            e = hipEventSynchronize(j->ev1);
+            if (e != hipSuccess && !rc) rc = hip_fail(e, "autotune walk chunk");
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, j->ev0, j->ev1);
+            if (rep == 1 && ms < best_chunk_ms) {
+                best_chunk_ms = ms;
+                win.walk_chunk = c;
+            }
+        }
+    }
     pfc_job_destroy(j);
     {
         std::lock_guard<std::mutex> g(dev->mu);

@@ -5,7 +5,10 @@
 // self-test, --device (accepted, ignored); factors.txt lines and final
 // checksum line (cl_sieve.cpp:565-577, 772-822); A/B checkpoint files with an
 // additive state sum (cl_sieve.cpp:104-243).  BOINC is not linked: the program
-// always behaves as the reference does standalone.
+// behaves as the reference does standalone, with the device ordinal taken
+// from the slot's init_data.xml when one is present (main.cpp:252-274);
+// trickle-up, fraction-done and critical sections have no client to talk to
+// and are not emitted.
 #include <getopt.h>
 #include <inttypes.h>
 #include <stdarg.h>
@@ -367,6 +370,33 @@ int run_test(pfc_device* dev, uint32_t threads) {
     return good == 16 ? EXIT_SUCCESS : EXIT_FAILURE;
 }
 
+// Device ordinal as boinc_get_opencl_ids finds it (main.cpp:252-274): the
+// <gpu_device_num> (else <gpu_opencl_dev_index>) of init_data.xml in the
+// working directory; without the file, device 0.
+int device_from_init_data() {
+    FILE* f = fopen("init_data.xml", "rb");
+    if (!f) {
+        say("init_data.xml not found, using device 0.\n");
+        return 0;
+    }
+    std::string xml;
+    char buf[4096];
+    size_t k;
+    while ((k = fread(buf, 1, sizeof(buf), f)) > 0) xml.append(buf, k);
+    fclose(f);
+    for (const char* tag : {"<gpu_device_num>", "<gpu_opencl_dev_index>"}) {
+        const size_t at = xml.find(tag);
+        if (at == std::string::npos) continue;
+        const long v = strtol(xml.c_str() + at + strlen(tag), nullptr, 10);
+        if (v >= 0 && v < 1024) {
+            say("init_data.xml: using device %ld.\n", v);
+            return (int)v;
+        }
+    }
+    say("init_data.xml has no GPU device number, using device 0.\n");
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -375,6 +405,7 @@ int main(int argc, char** argv) {
     if (!o.test && !o.st.factorial && !o.st.primorial && !o.st.compositorial)
         die("\n-! or -# or -c argument is required\nuse -h for help");
     if (const char* sc = getenv("PFC_SEG_CANDIDATES")) seg_candidates_env = (uint32_t)atol(sc);
+    o.device = device_from_init_data();
     pfc_device* dev = nullptr;
     check_rc(pfc_device_open(o.device, &dev));
     if (seg_candidates_env) {

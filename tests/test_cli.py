@@ -36,6 +36,24 @@ def test_argument_errors(tmp_path, args, msg):
     assert msg in r.stdout
 
 
+@pytest.mark.parametrize("xml,msg", [
+    (None, "init_data.xml not found, using device 0."),
+    ("<app_init_data>\n<gpu_device_num>3</gpu_device_num>\n</app_init_data>\n",
+     "init_data.xml: using device 3."),
+    ("<app_init_data><gpu_opencl_dev_index>2</gpu_opencl_dev_index></app_init_data>",
+     "init_data.xml: using device 2."),
+    ("<app_init_data></app_init_data>", "init_data.xml has no GPU device number, using device 0."),
+])
+def test_device_from_init_data(tmp_path, xml, msg):
+    """The BOINC slot's init_data.xml names the device (main.cpp:252-274); the
+    message is printed before the device is opened, so this runs without a GPU
+    (the run itself may then fail to open the device)."""
+    if xml is not None:
+        (tmp_path / "init_data.xml").write_text(xml)
+    r = run(["-!", "-p", "1e9", "-P", "1000001000", "-n", "101", "-N", "200"], tmp_path, timeout=120)
+    assert msg in r.stdout
+
+
 def test_help(tmp_path):
     r = run(["-h"], tmp_path)
     assert r.returncode != 0 and "Program usage" in r.stdout

@@ -331,3 +331,20 @@ def test_job_create_runs_table_check(dev):
         finally:
             dev.set_tuning()
     assert [r[1:] for r in res[:4]] == [r[1:] for r in res[4:]]
+
+
+@pytest.mark.gpu
+def test_autotune_then_run_matches_oracle(orc, dev):
+    """pfc_autotune picks a valid launch shape and n-chunk length, and a job
+    run with the tuned device gives the oracle's results."""
+    import pfcsieve_amd as PK
+    try:
+        t = dev.autotune(F | Cm, 10**9, 10**9 + 10**6, 101, 20000)
+        assert t["primes_per_lane"] in (1, 2, 4) and t["block_size"] in (64, 128, 256)
+        assert t["walk_chunk"] in (1 << 14, 1 << 16, 1 << 18)
+        assert dev.tuning() == t
+        r = PK.run_sieve(F | Cm, 10**9, 10**9 + 300000, 101, 20000, device=dev, verify_threads=4)
+        ref = orc.run(F | Cm, 10**9, 10**9 + 300000, 101, 20000)
+        assert (r.checksum, r.primecount, r.factorcount) == (ref.checksum, ref.primecount, ref.factorcount)
+    finally:
+        dev.set_tuning()
