@@ -348,3 +348,27 @@ def test_autotune_then_run_matches_oracle(orc, dev):
         assert (r.checksum, r.primecount, r.factorcount) == (ref.checksum, ref.primecount, ref.factorcount)
     finally:
         dev.set_tuning()
+
+
+@pytest.mark.gpu
+def test_wilson_and_prime_values_on_device(orc, dev):
+    """Properties fixed by the mathematics, checked on the device's output:
+    Wilson's theorem gives p | (p-1)!+1 and p | (p-2)!-1 for every prime p in
+    range; n!+1 (n = 116, 154, 320), n!-1 (n = 166, 324, 379), n#+1 (n = 379,
+    1019, 1021, 2657) and n#-1 (n = 317, 337, 991, 1873, 2053, 2377) are prime,
+    so no factor of them may be reported."""
+    import pfcsieve_amd as PK
+    r = PK.run_sieve(F, 3, 300000, 101, 2000, device=dev, verify_threads=8)
+    got = set((p, nc) for p, nc, t in r.factors)
+    for p in (int(x) for x in orc.primes(103, 2001)):
+        assert (p, p - 1) in got, p
+    for p in (int(x) for x in orc.primes(103, 2002)):
+        assert (p, -(p - 2)) in got, p
+    ncs = set(nc for _, nc, _ in r.factors)
+    assert not ncs & {116, 154, 320, -166, -324, -379}
+    rp = PK.run_sieve(P, 3, 300000, 101, 3000, device=dev, verify_threads=8)
+    ncs = set(nc for _, nc, _ in rp.factors)
+    assert rp.factorcount > 0
+    assert not ncs & {379, 1019, 1021, 2657, -317, -337, -991, -1873, -2053, -2377}
+    ref = orc.run(P, 3, 300000, 101, 3000)
+    assert (rp.checksum, rp.primecount, rp.factorcount) == (ref.checksum, ref.primecount, ref.factorcount)
