@@ -262,9 +262,10 @@ int pfc_build_setup_table(uint32_t kind, uint32_t N, uint64_t* chunks, size_t ch
  * table's start-residue path.  d_work: device buffer of PFC_VERIFY_WORDS u64;
  * afterwards d_work[0] = Mont(X) from the product and d_work[1] = the
  * table's start residue; the table is right when they are equal.
- * pfc_job_create runs this for every table it builds unless
- * pfc_tuning.skip_table_check is set, and fails with PFC_ERR_VALIDATION
- * ("power table verification failed") on a mismatch. */
+ * pfc_job_create queues this for every table it builds unless
+ * pfc_tuning.skip_table_check is set; the job's first finished segment
+ * (pfc_job_run or a batcher round) compares the results and fails with
+ * PFC_ERR_VALIDATION ("power table verification failed") on a mismatch. */
 #define PFC_VERIFY_BLOCKS 256
 #define PFC_VERIFY_WORDS (2 + PFC_VERIFY_BLOCKS)
 int pfc_stage_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_work,

@@ -339,6 +339,10 @@ struct pfc_job {
     pfc_ctrl* d_ctrl = nullptr;
     pfc_factor* d_fac = nullptr;
     pfc_ctrl* h_ctrl = nullptr;  // pinned
+    // start-table self-check, queued at creation and read at the first drain
+    uint64_t* d_verify = nullptr;   // 2 x PFC_VERIFY_WORDS
+    uint64_t* h_verify = nullptr;   // pinned: {product A, table A, product B, table B}
+    int verify_tables_n = 0;        // tables still to be checked (0: none pending)
     // captured segment pipeline (tuning.use_graph)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -359,6 +363,8 @@ void free_job_device(pfc_job* j) {
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (j->h_ctrl) hipHostFree(j->h_ctrl);
+    if (j->d_verify) (void)hipFree(j->d_verify);
+    if (j->h_verify) (void)hipHostFree(j->h_verify);
     if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
     if (j->graph) (void)hipGraphDestroy(j->graph);
     if (j->d_bounds) (void)hipFree(j->d_bounds);
@@ -433,30 +439,42 @@ uint64_t footprint_of(const pfc_job_desc* d, const pfc_tuning& t) {
 
 // Device self-check of the job's start tables against the product of their
 // multipliers modulo 2^64-59 (cl_sieve.cpp:917-970, 1079-1134, 1246-1304).
-int verify_tables(pfc_job* j) {
+// Queued on the job's stream at creation; the two results per table land in
+// pinned memory and are compared by check_tables when the job first
+// synchronises, so the check overlaps the host work that follows creation.
+int queue_table_check(pfc_job* j) {
     const bool two = (j->desc.modes & PFC_MODE_FACTORIAL) && (j->desc.modes & PFC_MODE_COMPOSITORIAL);
-    uint64_t* d_work = nullptr;
-    hipError_t e = hipMalloc((void**)&d_work, PFC_VERIFY_WORDS * sizeof(uint64_t));
-    if (e != hipSuccess) return set_error(PFC_ERR_NOMEM, "table check: device allocation failed");
+    hipError_t e = hipMalloc((void**)&j->d_verify, 2 * PFC_VERIFY_WORDS * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&j->h_verify, 4 * sizeof(uint64_t), hipHostMallocDefault);
+    if (e != hipSuccess) return set_error(PFC_ERR_NOMEM, "table check: allocation failed");
     const HostTable* hs[2] = {&j->ta, &j->tb};
     const uint64_t* cs[2] = {j->d_ca, j->d_cb};
     const uint32_t* gs[2] = {j->d_ga, j->d_gb};
-    int rc = PFC_OK;
-    for (int i = 0; i < (two ? 2 : 1) && !rc; ++i) {
+    const int n = two ? 2 : 1;
+    for (int i = 0; i < n; ++i) {
         const pfc_setup_table t{cs[i], gs[i], hs[i]->nbits, hs[i]->mont_k};
-        rc = pfc::launch_verify_table(hs[i]->kind, hs[i]->N, &t, d_work, j->stream);
-        uint64_t got[2] = {0, 1};
-        if (!rc) {
-            e = hipMemcpyAsync(got, d_work, sizeof got, hipMemcpyDeviceToHost, j->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(j->stream);
-            if (e != hipSuccess) rc = hip_fail(e, "table check");
-        }
-        if (!rc && got[0] != got[1])
-            rc = set_error(PFC_ERR_VALIDATION, hs[i]->kind == 0 ? "error: power table verification failed"
-                                                                : "error: product/prime table verification failed");
+        uint64_t* work = j->d_verify + (size_t)i * PFC_VERIFY_WORDS;
+        int rc = pfc::launch_verify_table(hs[i]->kind, hs[i]->N, &t, work, j->stream);
+        if (rc) return rc;
+        e = hipMemcpyAsync(j->h_verify + 2 * i, work, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, j->stream);
+        if (e != hipSuccess) return hip_fail(e, "table check");
     }
-    hipFree(d_work);
-    return rc;
+    j->verify_tables_n = n;
+    return PFC_OK;
+}
+
+// Compares the queued self-check results (waits for them if needed).
+int check_tables(pfc_job* j) {
+    if (!j->verify_tables_n) return PFC_OK;
+    hipError_t e = hipStreamSynchronize(j->stream);
+    if (e != hipSuccess) return hip_fail(e, "table check");
+    const HostTable* hs[2] = {&j->ta, &j->tb};
+    for (int i = 0; i < j->verify_tables_n; ++i)
+        if (j->h_verify[2 * i] != j->h_verify[2 * i + 1])
+            return set_error(PFC_ERR_VALIDATION, hs[i]->kind == 0 ? "error: power table verification failed"
+                                                                  : "error: product/prime table verification failed");
+    j->verify_tables_n = 0;
+    return PFC_OK;
 }
 
 }  // namespace
@@ -531,7 +549,7 @@ extern "C" int pfc_job_create(pfc_device* dev, const pfc_job_desc* desc, pfc_job
         e = hipStreamSynchronize(j->stream);
         if (e != hipSuccess) rc = hip_fail(e, "table upload");
     }
-    if (!rc && !j->tuning.skip_table_check) rc = verify_tables(j);
+    if (!rc && !j->tuning.skip_table_check) rc = queue_table_check(j);
     if (rc) {
         free_job_device(j);
         delete j;
@@ -635,6 +653,8 @@ static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
 
 static int drain_segment(pfc_job* j, bool* overflow, uint64_t launches, bool timed) {
     *overflow = false;
+    int rc = check_tables(j);
+    if (rc) return rc;
     hipError_t e;
     const pfc_ctrl c = *j->h_ctrl;
     if (c.flags & 1u) {
@@ -933,6 +953,8 @@ int ensure_device(T** p, size_t* cap, size_t need) {
 
 // Drains a finished segment into the job (factor records, sums, position).
 int finish_segment(pfc_job* j, const pfc_ctrl& c, uint64_t seg_hi) {
+    int rc = check_tables(j);
+    if (rc) return rc;
     if (c.flags & 2u) return set_error(PFC_ERR_VALIDATION, "error: gpu validation failure");
     if (c.nfactor > j->fcap) {
         char msg[128];
