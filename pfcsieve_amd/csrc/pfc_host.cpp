@@ -144,8 +144,13 @@ struct pfc_device {
     int ordinal = 0;
     hipDeviceProp_t prop{};
     pfc_tuning tuning{};
-    std::mutex mu;
+    mutable std::mutex mu;   // guards tuning
 };
+
+static pfc_tuning tuning_of(const pfc_device* dev) {
+    std::lock_guard<std::mutex> g(dev->mu);
+    return dev->tuning;
+}
 
 static pfc_tuning default_tuning() {
     pfc_tuning t;
@@ -212,7 +217,7 @@ extern "C" int pfc_set_tuning(pfc_device* dev, const pfc_tuning* t) {
 
 extern "C" int pfc_get_tuning(const pfc_device* dev, pfc_tuning* t) {
     if (!dev || !t) return set_error(PFC_ERR_INVALID, "get_tuning: null pointer");
-    *t = dev->tuning;
+    *t = tuning_of(dev);
     return PFC_OK;
 }
 
@@ -484,7 +489,7 @@ extern "C" int pfc_job_footprint(const pfc_device* dev, const pfc_job_desc* desc
     int rc = validate(desc);
     if (rc) return rc;
     if (!bytes) return set_error(PFC_ERR_INVALID, "footprint: null out");
-    *bytes = footprint_of(desc, dev ? dev->tuning : default_tuning());
+    *bytes = footprint_of(desc, dev ? tuning_of(dev) : default_tuning());
     return PFC_OK;
 }
 
@@ -790,7 +795,7 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
     const double lnp = log((double)std::max<uint64_t>(d.pmin, 16));
     const uint64_t want = (uint64_t)((double)(1u << 20) * lnp);
     if (d.pmax - d.pmin > want) d.pmax = d.pmin + want;
-    pfc_tuning saved = dev->tuning;
+    pfc_tuning saved = tuning_of(dev);
     pfc_tuning t = saved;
     t.seg_candidates = 1u << 20;
     {
@@ -800,6 +805,7 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
     pfc_job* j = nullptr;
     rc = pfc_job_create(dev, &d, &j);
     if (rc) {
+        std::lock_guard<std::mutex> g(dev->mu);
         dev->tuning = saved;
         return rc;
     }
@@ -867,7 +873,7 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
         std::lock_guard<std::mutex> g(dev->mu);
         dev->tuning = rc ? saved : win;
     }
-    if (best) *best = dev->tuning;
+    if (best) *best = tuning_of(dev);
     return rc;
 }
 
@@ -1064,7 +1070,11 @@ extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** d
     std::vector<pfc_batch_job> jobs(ns);
     struct Tmp { pfc_batch_item it; double cost; };
     std::vector<Tmp> items;
-    const uint32_t K = b->dev->tuning.primes_per_lane;
+    uint32_t K;
+    {
+        std::lock_guard<std::mutex> g(b->dev->mu);
+        K = b->dev->tuning.primes_per_lane;
+    }
     std::vector<uint8_t> finishing(ns, 0);
     for (size_t i = 0; i < ns; ++i) {
         pfc_batcher::Slot& s = b->slots[i];
@@ -1102,35 +1112,33 @@ extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** d
         if (rc) return rc;
     }
     // 3. drain finished segments
-    bool any_fin = false;
     for (size_t i = 0; i < ns; ++i) {
         if (!finishing[i]) continue;
         hipError_t e = hipMemcpyAsync(&b->h_ctrl[i], b->slots[i].job->d_ctrl, sizeof(pfc_ctrl),
                                       hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return hip_fail(e, "batcher ctrl read");
-        any_fin = true;
     }
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "batcher round");
     b->rounds += 1;
-    if (any_fin) {
-        std::vector<pfc_batcher::Slot> keep;
-        for (size_t i = 0; i < ns; ++i) {
-            pfc_batcher::Slot s = b->slots[i];
-            if (finishing[i]) {
-                rc = finish_segment(s.job, b->h_ctrl[i], s.seg_hi);
-                if (rc) return rc;
-                s.stage = s.job->pos >= s.job->desc.pmax ? 2 : 0;
-            }
-            if (s.stage == 2) {
-                if (done && *ndone < cap) done[*ndone] = s.job;
-                ++*ndone;
-            } else {
-                keep.push_back(s);
-            }
+    // Finished segments are drained; jobs with nothing left (including jobs
+    // added with their position already at pmax) are handed back.
+    std::vector<pfc_batcher::Slot> keep;
+    for (size_t i = 0; i < ns; ++i) {
+        pfc_batcher::Slot s = b->slots[i];
+        if (finishing[i]) {
+            rc = finish_segment(s.job, b->h_ctrl[i], s.seg_hi);
+            if (rc) return rc;
+            s.stage = s.job->pos >= s.job->desc.pmax ? 2 : 0;
         }
-        b->slots.swap(keep);
+        if (s.stage == 2) {
+            if (done && *ndone < cap) done[*ndone] = s.job;
+            ++*ndone;
+        } else {
+            keep.push_back(s);
+        }
     }
+    b->slots.swap(keep);
     return PFC_OK;
 }
 

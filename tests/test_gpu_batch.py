@@ -67,3 +67,25 @@ def test_continuous_admission(orc, dev):
         b.destroy()
         for j in jobs:
             j.destroy()
+
+
+def test_batch_returns_jobs_with_nothing_left(orc, dev):
+    """A job whose position is already at pmax (an empty p-range, or a job
+    that has been run to the end) is handed back by the next round, alone or
+    beside running jobs."""
+    import pfcsieve_amd as PK
+    empty = (F, 10**9, 10**9, 101, 500)
+    got = PK.run_batch([empty], device=dev, verify_threads=2)
+    assert (got[0].checksum, got[0].primecount, got[0].factorcount) == (0, 0, 0)
+    done_job = PK.Job(dev, *MIX[0])
+    done_job.run()
+    b = PK.Batcher(dev)
+    try:
+        b.add(done_job)
+        assert b.round() == [done_job] and b.pending() == 0
+    finally:
+        b.destroy()
+        done_job.destroy()
+    got = PK.run_batch([empty, MIX[2]], device=dev, verify_threads=8)
+    cs, pc, fac = _ref(orc, MIX[2])
+    assert (got[1].checksum, got[1].primecount, got[1].factors) == (cs, pc, fac)

@@ -372,3 +372,68 @@ def test_wilson_and_prime_values_on_device(orc, dev):
     assert not ncs & {379, 1019, 1021, 2657, -317, -337, -991, -1873, -2053, -2377}
     ref = orc.run(P, 3, 300000, 101, 3000)
     assert (rp.checksum, rp.primecount, rp.factorcount) == (ref.checksum, ref.primecount, ref.factorcount)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("modes,lo,width,nmin,nmax", [
+    (F | Cm, 10**12, 7_200_000, 101, 10**6),             # the headline shape: -! -c, n up to 1e6
+    (P, 2**62 - 4 * 10**9, 11_300_000, 101, 10**7),      # the primorial shape: n up to 1e7, p near 2^62
+])
+def test_full_size_walk_sampled_against_oracle(orc, dev, modes, lo, width, nmin, nmax):
+    """The full n-range of the benchmark configurations on a p-window of about
+    2^18 candidates, through the stage ABI (generate, set-up, walk in 65,536-step
+    launches); the final residues of 24 sampled candidates against the oracle's
+    own walk of each candidate over the same n-range."""
+    import random
+    import pfcsieve_amd._native as Nt
+    from pfcsieve_amd.sieve import DeviceBuffer
+    L = Nt.lib()
+    cap = 1 << 19
+    two = modes == (F | Cm)
+    bufs = {k: DeviceBuffer(dev, cap * w) for k, w in [("p", 8), ("v", 4), ("one", 8), ("ra", 8), ("rb", 8)]}
+    ctrl = DeviceBuffer(dev, 32)
+    ctrl.zero()
+    fac = DeviceBuffer(dev, 16 * 4096)
+    Nt.check(L.pfc_stage_generate(lo, lo + width, bufs["p"].ptr, cap, ctrl.ptr, None))
+    ta, keep_a = _table(dev, 1 if modes == P else 0, nmin - 1)
+    tb, keep_b = _table(dev, 2, nmin - 1)
+    Nt.check(L.pfc_stage_setup(bufs["p"].ptr, ctrl.ptr, bufs["v"].ptr, bufs["one"].ptr, bufs["ra"].ptr,
+                               bufs["rb"].ptr if two else None, C.byref(ta), C.byref(tb) if two else None,
+                               cap, None))
+    if modes == P:
+        steps = np.array([int(q) for q in orc.primes(nmin, nmax - 1)], np.uint32)
+        dlist = DeviceBuffer(dev, steps.nbytes)
+        dlist.upload(steps.tobytes())
+        j0, j1, lst, comp = 0, len(steps), dlist.ptr, None
+    else:
+        sv = np.ones(nmax, dtype=bool)
+        sv[:2] = False
+        for i in range(2, int(nmax ** 0.5) + 1):
+            if sv[i]:
+                sv[i * i::i] = False
+        bits = ~sv[nmin:nmax]
+        words = np.zeros((nmax - nmin) // 32 + 2, np.uint32)
+        idx = np.nonzero(bits)[0]
+        np.bitwise_or.at(words, idx >> 5, (np.uint32(1) << (idx & 31).astype(np.uint32)))
+        dcomp = DeviceBuffer(dev, words.nbytes)
+        dcomp.upload(words.tobytes())
+        j0, j1, lst, comp = nmin, nmax, None, dcomp.ptr
+    for a in range(j0, j1, 65536):
+        Nt.check(L.pfc_stage_walk(modes, a, min(j1, a + 65536), lst, comp, nmin, bufs["p"].ptr, bufs["v"].ptr,
+                                  bufs["one"].ptr, bufs["ra"].ptr, bufs["rb"].ptr if two else None, ctrl.ptr,
+                                  fac.ptr, 4096, cap, None))
+    Nt.check(L.pfc_device_synchronize(dev.handle))
+    u = np.frombuffer(ctrl.download(), np.uint32)
+    n, flags = int(u[0]), int(u[2])
+    assert flags & 1 == 0 and 200_000 < n <= cap
+    p = np.frombuffer(bufs["p"].download(8 * n), np.uint64)
+    ra = np.frombuffer(bufs["ra"].download(8 * n), np.uint64)
+    rb = np.frombuffer(bufs["rb"].download(8 * n), np.uint64) if two else None
+    rng = random.Random(modes * 7919 + nmax)
+    for i in rng.sample(range(n), 24):
+        pi = int(p[i])
+        s = 64 - pi.bit_length()
+        rec, _ = orc.candidate_record(modes, pi, nmin, nmax)
+        assert int(ra[i]) >> s == rec[6], (pi, "res_a")
+        if two:
+            assert int(rb[i]) >> s == rec[4], (pi, "res_b")
