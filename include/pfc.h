@@ -134,6 +134,16 @@ int pfc_job_stats_get(const pfc_job* job, pfc_job_stats* st);
 int pfc_job_factors(const pfc_job* job, pfc_factor* out, size_t cap, size_t* n);
 /* Drops the collected factor records (after they were harvested to disk). */
 int pfc_job_clear_factors(pfc_job* job);
+/* Waits for all work queued on the job's stream (the reference's sclFinish
+ * before reading results, cl_sieve.cpp:452).  pfc_job_run already returns
+ * with its segments complete; this also covers the queued table self-check. */
+int pfc_job_sync(pfc_job* job);
+/* getResults' device reads in one call (cl_sieve.cpp:452-501): the device
+ * checksum part (sum of check terms, before the n+c of kept factors is
+ * added), the candidate count, and the raw factor records sorted as
+ * pfc_job_factors returns them (out may be NULL to query *n). */
+int pfc_job_result(const pfc_job* job, uint64_t* checksum_partial, uint64_t* primecount, pfc_factor* out,
+                   size_t cap, size_t* n);
 int pfc_job_destroy(pfc_job* job);
 
 /* ------------------------------------------------------------------ batching */

@@ -76,6 +76,8 @@ SIGNATURES = {
     "pfc_job_stats_get": (_int, [_vp, C.POINTER(JobStats)]),
     "pfc_job_factors": (_int, [_vp, C.POINTER(Factor), _sz, C.POINTER(_sz)]),
     "pfc_job_clear_factors": (_int, [_vp]),
+    "pfc_job_sync": (_int, [_vp]),
+    "pfc_job_result": (_int, [_vp, C.POINTER(_u64), C.POINTER(_u64), _vp, _sz, C.POINTER(_sz)]),
     "pfc_job_destroy": (_int, [_vp]),
     "pfc_batch_run": (_int, [C.POINTER(_vp), _sz]),
     "pfc_batcher_create": (_int, [_vp, C.POINTER(_vp)]),

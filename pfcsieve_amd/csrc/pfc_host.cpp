@@ -767,6 +767,23 @@ extern "C" int pfc_job_factors(const pfc_job* j, pfc_factor* out, size_t cap, si
     return PFC_OK;
 }
 
+extern "C" int pfc_job_sync(pfc_job* j) {
+    pfc::g_err[0] = 0;
+    if (!j) return set_error(PFC_ERR_INVALID, "job_sync: null job");
+    hipSetDevice(j->dev->ordinal);
+    hipError_t e = hipStreamSynchronize(j->stream);
+    if (e != hipSuccess) return hip_fail(e, "job_sync");
+    return check_tables(j);
+}
+
+extern "C" int pfc_job_result(const pfc_job* j, uint64_t* checksum_partial, uint64_t* primecount,
+                              pfc_factor* out, size_t cap, size_t* n) {
+    if (!j || !checksum_partial || !primecount || !n) return set_error(PFC_ERR_INVALID, "job_result: null pointer");
+    *checksum_partial = j->dev_checksum;
+    *primecount = j->primecount;
+    return pfc_job_factors(j, out, cap, n);
+}
+
 extern "C" int pfc_job_clear_factors(pfc_job* j) {
     if (!j) return set_error(PFC_ERR_INVALID, "job_clear_factors: null job");
     j->factors.clear();

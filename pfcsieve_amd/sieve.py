@@ -151,6 +151,17 @@ class Job:
         check(lib().pfc_job_factors(self._h, arr, n.value, C.byref(n)))
         return arr, n.value
 
+    def sync(self):
+        check(lib().pfc_job_sync(self._h))
+
+    def result(self):
+        """(device checksum part, candidate count, raw factor records sorted) in one call."""
+        cs, pc, n = C.c_uint64(), C.c_uint64(), C.c_size_t()
+        check(lib().pfc_job_result(self._h, C.byref(cs), C.byref(pc), None, 0, C.byref(n)))
+        arr = (Factor * max(n.value, 1))()
+        check(lib().pfc_job_result(self._h, C.byref(cs), C.byref(pc), arr, n.value, C.byref(n)))
+        return cs.value, pc.value, [(f.p, f.nc, f.type) for f in arr[: n.value]]
+
     def clear_factors(self):
         check(lib().pfc_job_clear_factors(self._h))
 

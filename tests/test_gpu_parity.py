@@ -437,3 +437,24 @@ def test_full_size_walk_sampled_against_oracle(orc, dev, modes, lo, width, nmin,
         assert int(ra[i]) >> s == rec[6], (pi, "res_a")
         if two:
             assert int(rb[i]) >> s == rec[4], (pi, "res_b")
+
+
+@pytest.mark.gpu
+def test_job_result_and_sync(orc, dev):
+    """pfc_job_result returns the device checksum part, the candidate count
+    and the sorted raw records in one call, equal to the oracle's."""
+    import pfcsieve_amd as PK
+    job = (F | Cm, 10**6, 10**6 + 300000, 101, 3000)
+    j = PK.Job(dev, *job)
+    try:
+        j.sync()
+        j.run()
+        j.sync()
+        cs, pc, recs = j.result()
+        st = j.stats()
+        assert (cs, pc, len(recs)) == (st.dev_checksum, st.primecount, st.raw_factors)
+        ref = orc.run(*job)
+        assert (cs, pc, len(recs)) == (ref.dev_checksum, ref.primecount, ref.raw_factors)
+        assert recs == sorted(recs, key=lambda r: (r[0], abs(r[1]), r[2]))
+    finally:
+        j.destroy()
