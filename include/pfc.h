@@ -76,6 +76,8 @@ typedef struct pfc_tuning {
                                   walk chunks, check) once as a HIP graph and replay it per segment */
     uint32_t skip_table_check; /* 1: do not self-check the start tables at job creation
                                   (pfc_stage_verify_table); 0 (default): check them */
+    uint32_t lds_segment;      /* integers sieved per generator workgroup and staged in LDS:
+                                  15360 (default), 30720 or 61440 */
 } pfc_tuning;
 
 /* Host-side summary of a finished job (cl_sieve.cpp:1748). */
@@ -107,8 +109,9 @@ int pfc_set_tuning(pfc_device* dev, const pfc_tuning* t);
 int pfc_get_tuning(const pfc_device* dev, pfc_tuning* t);
 /* Project autotuner: times the walk kernel at every primes_per_lane x
  * block_size shape on a sample of `desc`, then the n-chunk length
- * (walk_chunk 2^14, 2^16, 2^18 steps per launch) at the fastest shape, and
- * stores the winners in the device tuning (also returned in *best).
+ * (walk_chunk 2^14, 2^16, 2^18 steps per launch) at the fastest shape, then
+ * the generator's LDS segment (lds_segment), and stores the winners in the
+ * device tuning (also returned in *best).
  * Counterpart of the reference's profile-driven sizing (cl_sieve.cpp:669-769,
  * 1642-1694). */
 int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tuning* best);

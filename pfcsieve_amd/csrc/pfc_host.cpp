@@ -160,6 +160,7 @@ static pfc_tuning default_tuning() {
     t.seg_candidates = 1u << 22;
     t.use_graph = 0;
     t.skip_table_check = 0;
+    t.lds_segment = 15360;   // 256 threads x one 60-integer window
     return t;
 }
 
@@ -210,6 +211,9 @@ extern "C" int pfc_set_tuning(pfc_device* dev, const pfc_tuning* t) {
     if (n.block_size != 64 && n.block_size != 128 && n.block_size != 256)
         return set_error(PFC_ERR_INVALID, "block_size must be 64, 128 or 256");
     if (n.seg_candidates > (1u << 28)) return set_error(PFC_ERR_CAPACITY, "seg_candidates > 2^28");
+    if (!n.lds_segment) n.lds_segment = d.lds_segment;
+    if (n.lds_segment != 15360 && n.lds_segment != 30720 && n.lds_segment != 61440)
+        return set_error(PFC_ERR_INVALID, "lds_segment must be 15360, 30720 or 61440");
     std::lock_guard<std::mutex> g(dev->mu);
     dev->tuning = n;
     return PFC_OK;
@@ -578,7 +582,7 @@ static int build_graph(pfc_job* j) {
     if (e != hipSuccess) return hip_fail(e, "graph bounds");
     e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-    int rc = pfc::launch_head_dev(j->d_bounds, j->plan.span, j->d_p, cap, j->d_ctrl, st);
+    int rc = pfc::launch_head_dev(j->d_bounds, j->plan.span, j->d_p, cap, j->d_ctrl, st, j->tuning.lds_segment);
     pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits, j->ta.mont_k};
     pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits, j->tb.mont_k};
     if (!rc)
@@ -629,7 +633,7 @@ static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
     *overflow = false;
     hipError_t e = hipMemsetAsync(j->d_ctrl, 0, sizeof(pfc_ctrl), st);
     if (e != hipSuccess) return hip_fail(e, "ctrl clear");
-    int rc = pfc::launch_generate(lo, hi, j->d_p, cap, j->d_ctrl, st);
+    int rc = pfc::launch_generate(lo, hi, j->d_p, cap, j->d_ctrl, st, j->tuning.lds_segment);
     if (rc) return rc;
     pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits, j->ta.mont_k};
     pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits, j->tb.mont_k};
@@ -830,7 +834,7 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
     const uint32_t cap = j->plan.cap;
     hipError_t e = hipMemsetAsync(j->d_ctrl, 0, sizeof(pfc_ctrl), st);
     if (e != hipSuccess) rc = hip_fail(e, "autotune clear");
-    if (!rc) rc = pfc::launch_generate(d.pmin, d.pmin + j->plan.span, j->d_p, cap, j->d_ctrl, st);
+    if (!rc) rc = pfc::launch_generate(d.pmin, d.pmin + j->plan.span, j->d_p, cap, j->d_ctrl, st, t.lds_segment);
     pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits, j->ta.mont_k};
     pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits, j->tb.mont_k};
     if (!rc)
@@ -882,6 +886,27 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
             if (rep == 1 && ms < best_chunk_ms) {
                 best_chunk_ms = ms;
                 win.walk_chunk = c;
+            }
+        }
+    }
+    // LDS segment of the generator: the sample window generated at each size.
+    const uint32_t segs[3] = {15360, 30720, 61440};
+    float best_gen_ms = 1e30f;
+    for (int rep = 0; rep < 2 && !rc; ++rep) {
+        for (uint32_t lds : segs) {
+            if (rc) break;
+            e = hipMemsetAsync(j->d_ctrl, 0, sizeof(pfc_ctrl), st);
+            if (e != hipSuccess) rc = hip_fail(e, "autotune clear");
+            (void)hipEventRecord(j->ev0, st);
+            if (!rc) rc = pfc::launch_generate(d.pmin, d.pmin + j->plan.span, j->d_p, cap, j->d_ctrl, st, lds);
+            (void)hipEventRecord(j->ev1, st);
+            e = hipEventSynchronize(j->ev1);
+            if (e != hipSuccess && !rc) rc = hip_fail(e, "autotune generate");
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, j->ev0, j->ev1);
+            if (rep == 1 && ms < best_gen_ms) {
+                best_gen_ms = ms;
+                win.lds_segment = lds;
             }
         }
     }
@@ -1060,7 +1085,7 @@ extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** d
         s.seg_hi = hi;
         hipError_t e = hipMemsetAsync(j->d_ctrl, 0, sizeof(pfc_ctrl), st);
         if (e != hipSuccess) return hip_fail(e, "batcher ctrl clear");
-        rc = pfc::launch_generate(j->pos, hi, j->d_p, j->plan.cap, j->d_ctrl, st);
+        rc = pfc::launch_generate(j->pos, hi, j->d_p, j->plan.cap, j->d_ctrl, st, j->tuning.lds_segment);
         if (rc) return rc;
         e = hipMemcpyAsync(&b->h_ctrl[i], j->d_ctrl, sizeof(pfc_ctrl), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return hip_fail(e, "batcher ctrl read");

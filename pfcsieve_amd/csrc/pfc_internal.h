@@ -41,10 +41,11 @@ extern thread_local char g_err[512];
 int set_error(int code, const char* msg);
 int hip_fail(hipError_t e, const char* what);
 
+// lds_segment: integers sieved per generator workgroup (15360, 30720, 61440; 0 = 15360).
 int launch_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pfc_ctrl* d_ctrl,
-                    hipStream_t st);
+                    hipStream_t st, uint32_t lds_segment);
 int launch_head_dev(const void* d_bounds, uint64_t max_span, uint64_t* d_p, uint32_t cap,
-                    pfc_ctrl* d_ctrl, hipStream_t st);
+                    pfc_ctrl* d_ctrl, hipStream_t st, uint32_t lds_segment);
 int launch_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uint64_t* d_one,
                  uint64_t* d_ra, uint64_t* d_rb, const pfc_setup_table* a,
                  const pfc_setup_table* b, uint32_t cap, hipStream_t st);

@@ -30,8 +30,8 @@ namespace {
 // ------------------------------------------------------------------ generate
 
 constexpr int kGenBlock = 256;
-constexpr int kGenSpan = 60;                      // integers per thread
-constexpr int kGenLds = kGenBlock * 16;           // at most 16 survivors per thread
+constexpr int kGenSpan = 60;                      // integers per window (one 64-bit mask)
+constexpr int kGenLdsPerWindow = kGenBlock * 16;  // at most 16 survivors per window
 
 // Appends values to out[] with one atomic per wavefront.
 __device__ __forceinline__ void wave_append(bool keep, uint64_t value, uint64_t* out, uint32_t cap,
@@ -51,28 +51,33 @@ __device__ __forceinline__ void wave_append(bool keep, uint64_t value, uint64_t*
     }
 }
 
-// Block b covers [base30 + b*15360, +15360), thread t the 60 integers from
-// base30 + b*15360 + 60*t; only integers in [lo, hi) are kept.  Survivors of
-// the wheel and the 7..113 sieve are compacted in LDS, then PRP-tested by the
-// whole block (the reference's shape, getsegprimes.cl:477-517, with the LDS
-// list bounded by construction: at most 16 survivors per thread).
+// The LDS segment of one workgroup is W*15360 integers: block b covers
+// [base30 + b*W*15360, +W*15360), and window w of thread t the 60 integers
+// from base30 + b*W*15360 + 60*(w*256 + t); only integers in [lo, hi) are
+// kept.  Survivors of the wheel and the 7..113 sieve are compacted in LDS,
+// then PRP-tested by the whole block (the reference's shape,
+// getsegprimes.cl:477-517, with the LDS list bounded by construction: at most
+// 16 survivors per window).  W (1, 2, 4) is the tunable LDS segment size.
+template <int W>
 __device__ __forceinline__ void generate_block(uint64_t base30, uint64_t lo, uint64_t hi,
                                                uint64_t* out, uint32_t cap, pfc_ctrl* ctrl) {
-    __shared__ uint64_t s_list[kGenLds];
+    __shared__ uint64_t s_list[kGenLdsPerWindow * W];
     __shared__ uint32_t s_mod[27];
     __shared__ uint32_t s_count;
     const uint32_t tid = threadIdx.x;
-    const uint64_t block_base = base30 + (uint64_t)blockIdx.x * (kGenBlock * kGenSpan);
+    const uint64_t block_base = base30 + (uint64_t)blockIdx.x * (kGenBlock * kGenSpan * W);
     if (tid < 27) s_mod[tid] = (uint32_t)(block_base % pfc::kSievePrimes[tid]);
     if (tid == 0) s_count = 0;
     __syncthreads();
 
-    const uint64_t start = block_base + (uint64_t)tid * kGenSpan;
-    uint32_t rmod[27];
 #pragma unroll
-    for (int i = 0; i < 27; ++i) rmod[i] = (s_mod[i] + tid * kGenSpan) % pfc::kSievePrimes[i];
-    uint64_t cand = pfc::window_survivors(start, lo, hi, rmod);
-    if (cand) {
+    for (int w = 0; w < W; ++w) {
+        const uint32_t off = (uint32_t)(w * kGenBlock + tid) * kGenSpan;
+        const uint64_t start = block_base + off;
+        uint32_t rmod[27];
+#pragma unroll
+        for (int i = 0; i < 27; ++i) rmod[i] = (s_mod[i] + off) % pfc::kSievePrimes[i];
+        uint64_t cand = pfc::window_survivors(start, lo, hi, rmod);
         while (cand) {
             const uint32_t j = __ffsll((unsigned long long)cand) - 1;
             cand &= cand - 1;
@@ -94,9 +99,10 @@ __device__ __forceinline__ void generate_block(uint64_t base30, uint64_t lo, uin
     }
 }
 
+template <int W>
 __global__ __launch_bounds__(kGenBlock) void k_generate(uint64_t base30, uint64_t lo, uint64_t hi,
                                                        uint64_t* out, uint32_t cap, pfc_ctrl* ctrl) {
-    generate_block(base30, lo, hi, out, cap, ctrl);
+    generate_block<W>(base30, lo, hi, out, cap, ctrl);
 }
 
 // Segment bounds in device memory, so that one captured graph serves every
@@ -105,12 +111,13 @@ struct SegBounds {
     uint64_t lo, hi;
 };
 
+template <int W>
 __global__ __launch_bounds__(kGenBlock) void k_generate_dev(const SegBounds* sb, uint64_t* out,
                                                            uint32_t cap, pfc_ctrl* ctrl) {
     const uint64_t lo = sb->lo < 114 ? 114 : sb->lo;
     const uint64_t hi = sb->hi;
     if (hi <= lo) return;
-    generate_block(lo - lo % 30, lo, hi, out, cap, ctrl);
+    generate_block<W>(lo - lo % 30, lo, hi, out, cap, ctrl);
 }
 
 __global__ void k_zero_ctrl(pfc_ctrl* ctrl) {
@@ -656,7 +663,9 @@ int hip_fail(hipError_t e, const char* what) {
 }
 
 int launch_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pfc_ctrl* d_ctrl,
-                    hipStream_t st) {
+                    hipStream_t st, uint32_t lds_segment) {
+    const uint32_t W = lds_segment ? lds_segment / (kGenBlock * kGenSpan) : 1;
+    if (W != 1 && W != 2 && W != 4) return set_error(PFC_ERR_INVALID, "lds_segment must be 15360, 30720 or 61440");
     if (hi <= lo) return PFC_OK;
     if (lo < 114) {
         const uint64_t h = hi < 114 ? hi : 114;
@@ -667,11 +676,18 @@ int launch_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pfc_c
     }
     const uint64_t base30 = lo - lo % 30;
     const uint64_t span = hi - base30;
-    const uint64_t per_block = (uint64_t)kGenBlock * kGenSpan;
+    const uint64_t per_block = (uint64_t)kGenBlock * kGenSpan * W;
     const uint64_t blocks = (span + per_block - 1) / per_block;
     if (blocks > 0x7FFFFFFFull) return set_error(PFC_ERR_CAPACITY, "segment too wide");
-    hipLaunchKernelGGL(k_generate, dim3((uint32_t)blocks), dim3(kGenBlock), 0, st, base30, lo, hi,
-                       d_p, cap, d_ctrl);
+    if (W == 1)
+        hipLaunchKernelGGL(k_generate<1>, dim3((uint32_t)blocks), dim3(kGenBlock), 0, st, base30, lo, hi, d_p,
+                           cap, d_ctrl);
+    else if (W == 2)
+        hipLaunchKernelGGL(k_generate<2>, dim3((uint32_t)blocks), dim3(kGenBlock), 0, st, base30, lo, hi, d_p,
+                           cap, d_ctrl);
+    else
+        hipLaunchKernelGGL(k_generate<4>, dim3((uint32_t)blocks), dim3(kGenBlock), 0, st, base30, lo, hi, d_p,
+                           cap, d_ctrl);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_generate launch");
 }
@@ -680,14 +696,21 @@ int launch_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pfc_c
 // a kernel (no memset node), primes below 114, and the generator sized for
 // segments up to max_span integers, all reading bounds from d_bounds.
 int launch_head_dev(const void* d_bounds, uint64_t max_span, uint64_t* d_p, uint32_t cap,
-                    pfc_ctrl* d_ctrl, hipStream_t st) {
+                    pfc_ctrl* d_ctrl, hipStream_t st, uint32_t lds_segment) {
+    const uint32_t W = lds_segment ? lds_segment / (kGenBlock * kGenSpan) : 1;
+    if (W != 1 && W != 2 && W != 4) return set_error(PFC_ERR_INVALID, "lds_segment must be 15360, 30720 or 61440");
     const SegBounds* sb = (const SegBounds*)d_bounds;
     hipLaunchKernelGGL(k_zero_ctrl, dim3(1), dim3(64), 0, st, d_ctrl);
     hipLaunchKernelGGL(k_small_primes, dim3(1), dim3(64), 0, st, 0ull, 0ull, d_p, cap, d_ctrl, sb);
-    const uint64_t per_block = (uint64_t)kGenBlock * kGenSpan;
+    const uint64_t per_block = (uint64_t)kGenBlock * kGenSpan * W;
     const uint64_t blocks = (max_span + 60 + per_block - 1) / per_block;
     if (blocks > 0x7FFFFFFFull) return set_error(PFC_ERR_CAPACITY, "segment too wide");
-    hipLaunchKernelGGL(k_generate_dev, dim3((uint32_t)blocks), dim3(kGenBlock), 0, st, sb, d_p, cap, d_ctrl);
+    if (W == 1)
+        hipLaunchKernelGGL(k_generate_dev<1>, dim3((uint32_t)blocks), dim3(kGenBlock), 0, st, sb, d_p, cap, d_ctrl);
+    else if (W == 2)
+        hipLaunchKernelGGL(k_generate_dev<2>, dim3((uint32_t)blocks), dim3(kGenBlock), 0, st, sb, d_p, cap, d_ctrl);
+    else
+        hipLaunchKernelGGL(k_generate_dev<4>, dim3((uint32_t)blocks), dim3(kGenBlock), 0, st, sb, d_p, cap, d_ctrl);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "graph head launch");
 }
@@ -851,7 +874,7 @@ int pfc_stage_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pf
                        void* stream) {
     pfc::g_err[0] = 0;
     if (!d_p || !d_ctrl) return pfc::set_error(PFC_ERR_INVALID, "generate: null pointer");
-    return pfc::launch_generate(lo, hi, d_p, cap, d_ctrl, (hipStream_t)stream);
+    return pfc::launch_generate(lo, hi, d_p, cap, d_ctrl, (hipStream_t)stream, 0);
 }
 
 int pfc_stage_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uint64_t* d_one,

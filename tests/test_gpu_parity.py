@@ -342,6 +342,7 @@ def test_autotune_then_run_matches_oracle(orc, dev):
         t = dev.autotune(F | Cm, 10**9, 10**9 + 10**6, 101, 20000)
         assert t["primes_per_lane"] in (1, 2, 4) and t["block_size"] in (64, 128, 256)
         assert t["walk_chunk"] in (1 << 14, 1 << 16, 1 << 18)
+        assert t["lds_segment"] in (15360, 30720, 61440)
         assert dev.tuning() == t
         r = PK.run_sieve(F | Cm, 10**9, 10**9 + 300000, 101, 20000, device=dev, verify_threads=4)
         ref = orc.run(F | Cm, 10**9, 10**9 + 300000, 101, 20000)
@@ -458,3 +459,20 @@ def test_job_result_and_sync(orc, dev):
         assert recs == sorted(recs, key=lambda r: (r[0], abs(r[1]), r[2]))
     finally:
         j.destroy()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lds", [30720, 61440])
+@pytest.mark.parametrize("job", [(F, 101, 200000, 101, 600), (Cm, 10**12, 10**12 + 10**6, 101, 500),
+                                 (P, TWO64 - 3 * 10**6, TWO64 - 1, 101, 400)])
+def test_generator_lds_segments(orc, dev, lds, job):
+    """Jobs run with the larger generator LDS segments give the oracle's
+    results (candidate set, sums and factors)."""
+    import pfcsieve_amd as PK
+    dev.set_tuning(lds_segment=lds)
+    try:
+        got = PK.run_sieve(*job, device=dev, verify_threads=8)
+    finally:
+        dev.set_tuning()
+    ref = orc.run(*job)
+    assert (got.checksum, got.primecount, got.factorcount) == (ref.checksum, ref.primecount, ref.factorcount)
