@@ -241,6 +241,25 @@ def other_configs(dev):
     return out
 
 
+def shape_sweep(dev, modes, pmin, nmin, nmax):
+    """Walk rate of one 2^21-candidate headline segment at every launch shape
+    the autotuner considers (primes per lane x workgroup size)."""
+    import pfcsieve_amd as P
+    out = []
+    for ppl in (1, 2, 4):
+        for blk in (64, 128, 256):
+            dev.set_tuning(primes_per_lane=ppl, block_size=blk, seg_candidates=1 << 21)
+            j = P.Job(dev, modes, pmin, pmin + 10**9, nmin, nmax)
+            j.run(1)
+            s = j.stats()
+            j.destroy()
+            out.append({"primes_per_lane": ppl, "block_size": blk,
+                        "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)})
+            _progress(True, f"sweep ppl={ppl} block={blk}: {out[-1]['walk_updates_per_s']:.4e} updates/s")
+    dev.set_tuning()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +279,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run one plain copy of this many bytes first (HBM counter calibration)")
     ap.add_argument("--progress", action="store_true", help="progress lines on stderr during long mixes")
+    ap.add_argument("--sweep", action="store_true", help="walk rate at every (primes per lane, block) shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding over p is "
@@ -326,6 +346,9 @@ def main():
     }
     if args.configs:
         line["other_configs"] = other_configs(dev)
+    if args.sweep:
+        line["shape_sweep"] = shape_sweep(dev, modes, pmin, nmin, nmax)
+        dev.set_tuning(seg_candidates=args.seg_candidates, primes_per_lane=args.ppl, block_size=args.block)
     if args.mix_units:
         from pfcsieve_amd import workload as W
         units = W.mix(256, seed=12345)[: args.mix_units]
