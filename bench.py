@@ -110,10 +110,11 @@ def mix_compare(dev, units, verify_threads=16, progress=False):
         j.destroy()
         _progress(progress, f"plain: unit {len(cand_plain)}/{len(units)} done, {time.perf_counter() - t:.1f} s")
     t_plain = time.perf_counter() - t
-    # batched
+    # batched (job creation -- host tables, device state -- inside the timed
+    # region, as it is for the plain form)
+    t = time.perf_counter()
     jobs = [P.Job(dev, *u.desc()) for u in units]
     b = P.Batcher(dev)
-    t = time.perf_counter()
     for j in jobs:
         b.add(j)
     rounds = 0
@@ -121,12 +122,12 @@ def mix_compare(dev, units, verify_threads=16, progress=False):
         b.round(0)
         rounds += 1
         _progress(progress, f"batched: round {rounds} done, {b.pending()} jobs pending, {time.perf_counter() - t:.1f} s")
-    t_batch = time.perf_counter() - t
     cand_batch = [j.stats().primecount for j in jobs]
     sig_batch = [(j.stats().dev_checksum, j.stats().primecount, j.stats().raw_factors) for j in jobs]
     for j in jobs:
         j.destroy()
     b.destroy()
+    t_batch = time.perf_counter() - t
     assert cand_plain == cand_batch
     upd = sum(c * s for c, s in zip(cand_plain, steps))
     return {"units": len(units), "candidates": sum(cand_plain), "walk_updates": upd,
