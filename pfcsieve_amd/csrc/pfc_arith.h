@@ -84,45 +84,57 @@ PFC_HD uint64_t div3by2_rem(uint32_t u2, uint32_t u1, uint32_t u0, const Modulus
 // correction: returns x with x == (a*n mod d) or x == (a*n mod d) + d, and
 // sets *fix when the caller must subtract d (x >= d).  The walk kernel uses
 // this form so that the rare correction becomes a wavefront-uniform branch.
-// nd1 = 2^32 - d1 (the high half of d, negated): u1 - q1*d1 then becomes the
-// low word of one 32x32+64 multiply-add whose 64-bit addend is <u2,u1> as
-// already computed, so no register pair has to be assembled for it.
-PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t nd1, uint32_t d0, uint64_t d,
+//
+// The 3-by-2 step forms r = <u1,u0> - (q1 + 1) * d (mod 2^64) with per-modulus
+// constants instead of a borrow chain: with k1 = -(d1 + 1) and k0 = 2^32 - d0
+// (k1 = -d1 and k0 = 0 when d0 == 0) and nd = 2^64 - d,
+//   r = q1 * k0 + (<u1 + q1 * k1, u0> + nd)      (mod 2^64),
+// since q1 * k0 = q1 * 2^32 - q1 * d0 and the extra q1 * 2^32 cancels the -q1
+// folded into k1.  u1 + q1 * k1 is the low word of one multiply-add whose
+// 64-bit addend is <u2,u1> as computed, and the two remaining terms are one
+// 64-bit add and one multiply-add.
+PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t k1, uint32_t k0, uint64_t nd, uint64_t d,
                                uint32_t v, bool* fix) {
     const uint64_t lo = (uint64_t)(uint32_t)a * n;
     const uint64_t hi = (uint64_t)(uint32_t)(a >> 32) * n + (lo >> 32);   // <u2,u1>
-    const uint32_t u0 = (uint32_t)lo, u1 = (uint32_t)hi, u2 = (uint32_t)(hi >> 32);
+    const uint32_t u2 = (uint32_t)(hi >> 32);
     const uint64_t qq = (uint64_t)v * u2 + hi;
     const uint32_t q1 = (uint32_t)(qq >> 32), q0 = (uint32_t)qq;
-    const uint64_t t = (uint64_t)d0 * q1 + d;
-    // <u1 - q1*d1, u0> - t with an explicit borrow chain across the halves
-    // (v_sub_co / v_subb_co on the device).
-    uint32_t borrow, borrow2;
-    const uint32_t xlo = subb32(u0, (uint32_t)t, 0u, &borrow);
-    const uint32_t w = (uint32_t)((uint64_t)q1 * nd1 + hi);   // u1 - q1*d1 (mod 2^32)
-    const uint32_t xhi = subb32(w, (uint32_t)(t >> 32), borrow, &borrow2);
-    (void)borrow2;
-    uint64_t x = ((uint64_t)xhi << 32) | xlo;
+    const uint32_t w = (uint32_t)((uint64_t)q1 * k1 + hi);              // u1 - q1 * (d1 + 1)
+    const uint64_t pair = (((uint64_t)w) << 32) | (uint32_t)lo;        // <w, u0>
+#if defined(__HIP_DEVICE_COMPILE__)
+    // One 64-bit add on the assembled pair.  Written as an instruction so the
+    // optimiser cannot split the pair and reassociate the sum into two adds.
+    uint64_t y;
+    asm("v_lshl_add_u64 %0, %1, 0, %2" : "=v"(y) : "v"(pair), "v"(nd));
+#else
+    const uint64_t y = pair + nd;
+#endif
+    uint64_t x = (uint64_t)q1 * k0 + y;
 #if defined(__HIP_DEVICE_COMPILE__)
     // An exec-masked add instead of two selects: the empty volatile asm keeps
     // the compiler from turning the branch into v_cndmask pairs, so the
     // correction costs one VALU add plus scalar exec-mask bookkeeping.
-    if (xhi >= q0) {
+    if ((uint32_t)(x >> 32) >= q0) {
         asm volatile("" ::: );
         x += d;
     }
 #else
-    if (xhi >= q0) x += d;
+    if ((uint32_t)(x >> 32) >= q0) x += d;
 #endif
     *fix = x >= d;
     return x;
 }
 
+// The constants k1, k0 of mulmod32_nofix for d = <d1, d0>.
+PFC_HD uint32_t walk_k1(uint32_t d1, uint32_t d0) { return d0 ? ~d1 : 0u - d1; }
+PFC_HD uint32_t walk_k0(uint32_t d0) { return 0u - d0; }
+
 // (a * n) mod d for a < d and any 32-bit n.  Keeps the shifted form.
 PFC_HD uint64_t mulmod32(uint64_t a, uint32_t n, const Modulus& m) {
     bool fix;
     const uint64_t d = mod_d(m);
-    const uint64_t x = mulmod32_nofix(a, n, 0u - m.d1, m.d0, d, m.v, &fix);
+    const uint64_t x = mulmod32_nofix(a, n, walk_k1(m.d1, m.d0), walk_k0(m.d0), 0ull - d, d, m.v, &fix);
     return fix ? x - d : x;
 }
 

@@ -266,8 +266,8 @@ struct WalkArgs {
 // steps.
 template <int K>
 struct Lanes {
-    uint32_t nd1[K], d0[K], v[K], s[K];   // nd1 = 2^32 - (d >> 32)
-    uint64_t d[K], one[K], nmo[K];
+    uint32_t k1[K], k0[K], v[K], s[K];    // k1, k0: pfc::walk_k1 / walk_k0 of d
+    uint64_t d[K], nd[K], one[K], nmo[K]; // nd = 2^64 - d
     bool live[K];
 };
 
@@ -276,8 +276,9 @@ __device__ __forceinline__ void lanes_from_state(Lanes<K>& L, uint64_t p, uint32
                                                  int k) {
     L.s[k] = pfc::clz64(p);
     L.d[k] = p << L.s[k];
-    L.nd1[k] = 0u - (uint32_t)(L.d[k] >> 32);
-    L.d0[k] = (uint32_t)L.d[k];
+    L.k1[k] = pfc::walk_k1((uint32_t)(L.d[k] >> 32), (uint32_t)L.d[k]);
+    L.k0[k] = pfc::walk_k0((uint32_t)L.d[k]);
+    L.nd[k] = 0ull - L.d[k];
     L.v[k] = v;
     L.one[k] = one;
     L.nmo[k] = L.d[k] - one;
@@ -293,7 +294,7 @@ __device__ __forceinline__ bool step(uint64_t (&r)[K], uint32_t n, const Lanes<K
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         bool f;
-        r[k] = pfc::mulmod32_nofix(r[k], n, L.nd1[k], L.d0[k], L.d[k], L.v[k], &f);
+        r[k] = pfc::mulmod32_nofix(r[k], n, L.k1[k], L.k0[k], L.nd[k], L.d[k], L.v[k], &f);
         fix |= f;
     }
     // The second correction of the 3-by-2 division is rare: take it as a
@@ -578,8 +579,8 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
         }
         if (it.flags & kBatchCheck) {
             Modulus m;
-            m.d1 = 0u - L.nd1[k];
-            m.d0 = L.d0[k];
+            m.d1 = (uint32_t)(L.d[k] >> 32);
+            m.d0 = (uint32_t)L.d[k];
             m.v = L.v[k];
             m.s = L.s[k];
             sum += check_term(J.modes, J.nmax, m, L.one[k], ra[k], rb[k]);
