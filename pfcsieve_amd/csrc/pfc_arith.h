@@ -313,8 +313,21 @@ PFC_HD uint64_t redc_mul(uint64_t a, uint64_t b, uint64_t p, uint64_t q) {
     const uint64_t lo = (p10 << 32) | (uint32_t)p00;
     const uint64_t m = lo * q;
     const uint64_t mp = mulhi64(m, p);
-    const uint64_t r = hi - mp;
-    return hi < mp ? r + p : r;
+    // hi - mp with an explicit borrow chain whose final borrow is the
+    // correction condition (one exec-masked add on the device, no compare).
+    uint32_t br0, br1;
+    const uint32_t rlo = subb32((uint32_t)hi, (uint32_t)mp, 0u, &br0);
+    const uint32_t rhi = subb32((uint32_t)(hi >> 32), (uint32_t)(mp >> 32), br0, &br1);
+    uint64_t r = ((uint64_t)rhi << 32) | rlo;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (br1) {
+        asm volatile("" ::: );
+        r += p;
+    }
+#else
+    if (br1) r += p;
+#endif
+    return r;
 }
 
 // Start residue (X * 2^64 mod p) << s of X = prod over groups b (high first)
