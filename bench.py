@@ -68,19 +68,19 @@ def cpu_baseline(modes, p0, width, nmin, nmax, steps_per_cand, width3=600_000):
            "sample": f"-! -c p in [{p0}, {p0 + width}) ({r.primecount} candidates), n in [{nmin}, {nmax}); "
                      f"{dt:.1f} s",
            "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
-    # cfg1 in full: -! -n 101 -N 5000 -p 1e9 -P 1.001e9
+    # the reference's CPU-runnable case in full: -! -n 101 -N 5000 -p 1e9 -P 1.001e9
     t = time.perf_counter()
     r1 = O.run(1, 10**9, 10**9 + 10**6, 101, 5000)
     dt1 = time.perf_counter() - t
-    out["cfg1_full"] = {"candidates": r1.primecount, "s": dt1, "updates_per_s": r1.primecount * 4899 / dt1,
+    out["factorial_n5000_p1e9_full"] = {"candidates": r1.primecount, "s": dt1, "updates_per_s": r1.primecount * 4899 / dt1,
                         "checksum": "%016X" % r1.checksum}
-    # cfg3 p-sample: -# -n 101 -N 1e7 from p = 2^62 - 4e9
+    # primorial p-sample: -# -n 101 -N 1e7 from p = 2^62 - 4e9
     p3 = 2**62 - 4 * 10**9
     steps3 = W.walk_steps(W.Unit(2, 0, 0, 101, 10**7))
     t = time.perf_counter()
     r3 = O.run(2, p3, p3 + width3, 101, 10**7)
     dt3 = time.perf_counter() - t
-    out["cfg3_sample"] = {"sample": f"-# p in [{p3}, {p3 + width3}), n in [101, 1e7)",
+    out["primorial_n1e7_p2e62_sample"] = {"sample": f"-# p in [{p3}, {p3 + width3}), n in [101, 1e7)",
                           "candidates": r3.primecount, "s": dt3,
                           "updates_per_s": r3.primecount * steps3 / dt3}
     return out
@@ -188,18 +188,19 @@ def verify_baseline(dev):
 
 
 def other_configs(dev):
-    """Throughput on the other single-GPU configurations: cfg1 (the reference's
-    CPU-runnable case, whole job) and cfg3 (primorial near 2^62, one p-segment)."""
+    """Throughput on the other single-GPU workloads: the reference's CPU-runnable
+    case (-! n [101,5000) p [1e9,1.001e9), whole job), the primorial walk near
+    2^62 (-# n [101,1e7), one p-segment), single-mode walks and large-nmin set-up."""
     import pfcsieve_amd as P
     out = {}
-    # cfg1: -! -n 101 -N 5000 -p 1e9 -P 1.001e9 (48,155 candidates)
+    # -! -n 101 -N 5000 -p 1e9 -P 1.001e9 (48,155 candidates), whole job
     t = time.perf_counter()
     r = P.run_sieve(1, 10**9, 10**9 + 10**6, 101, 5000, device=dev, verify_threads=16)
     dt = time.perf_counter() - t
-    out["cfg1"] = {"workload": "-! n [101,5000) p [1e9,1.001e9)", "candidates": r.primecount,
+    out["factorial_n5000_p1e9"] = {"workload": "-! n [101,5000) p [1e9,1.001e9)", "candidates": r.primecount,
                    "updates": r.primecount * 4899, "wall_s": dt, "updates_per_s": r.primecount * 4899 / dt,
                    "factors": r.factorcount, "checksum": "%016X" % r.checksum}
-    # cfg3: -# -n 101 -N 1e7 -p 2^62-4e9 -P 2^62, one segment
+    # -# -n 101 -N 1e7 -p 2^62-4e9 -P 2^62, one segment
     from pfcsieve_amd import workload as W
     steps = W.walk_steps(W.Unit(2, 0, 0, 101, 10**7))
     j = P.Job(dev, 2, 2**62 - 4 * 10**9, 2**62, 101, 10**7)
@@ -208,7 +209,7 @@ def other_configs(dev):
     dt = time.perf_counter() - t
     s = j.stats()
     j.destroy()
-    out["cfg3_segment"] = {"workload": "-# n [101,1e7) p from 2^62-4e9, one p-segment", "candidates": s.primecount,
+    out["primorial_n1e7_p2e62_segment"] = {"workload": "-# n [101,1e7) p from 2^62-4e9, one p-segment", "candidates": s.primecount,
                            "steps_per_candidate": steps, "wall_s": dt,
                            "updates_per_s": s.primecount * steps / dt,
                            "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
@@ -274,7 +275,7 @@ def main():
     ap.add_argument("--mix-units", type=int, default=24,
                     help="units of the seeded 256-unit batched mix to compare plain vs batched (0 = skip)")
     ap.add_argument("--autotune", action="store_true")
-    ap.add_argument("--configs", action="store_true", help="also measure cfg1 and one cfg3 segment")
+    ap.add_argument("--configs", action="store_true", help="also measure the other single-GPU workloads (whole small job, primorial segment, single modes, set-up)")
     ap.add_argument("--serve-jobs", type=int, default=0,
                     help="jobs of the seeded served stream to run through the serving loop (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -284,7 +285,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding over p is "
-                                   "described in DESIGN.md but not built", "n_gpus": args.gpus}))
+                                   "is a design only, not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import pfcsieve_amd as P

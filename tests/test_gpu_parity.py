@@ -37,9 +37,9 @@ def gen_on_gpu(dev, lo, hi, cap=None):
     (3, 100000),                       # primes below 114 plus the sieve
     (101, 114),                        # only addsmallprimes territory
     (114, 120),                        # empty-ish window
-    (10**9, 10**9 + 10**6),            # cfg1 window: 48,155 candidates
+    (10**9, 10**9 + 10**6),            # the reference's CPU-runnable window: 48,155 candidates
     ((1 << 32) - 300000, (1 << 32) + 300000),
-    ((1 << 62) - 4 * 10**6, 1 << 62),  # cfg3 carry edge, scaled down
+    ((1 << 62) - 4 * 10**6, 1 << 62),  # just below 2^62
     (TWO64 - 2 * 10**6, TWO64 - 1),    # top of the 64-bit range
 ])
 def test_generate_matches_oracle(orc, dev, lo, hi):
@@ -49,7 +49,7 @@ def test_generate_matches_oracle(orc, dev, lo, hi):
     assert np.array_equal(got, np.sort(ref))
 
 
-def test_generate_cfg1_count(orc, dev):
+def test_generate_factorial_window_count(orc, dev):
     got = gen_on_gpu(dev, 10**9, 10**9 + 10**6)
     assert len(got) == 48155
 
@@ -60,7 +60,7 @@ SMALL_JOBS = [
     (Cm, 101, 30000, 101, 4000),
     (F | Cm, 101, 30000, 101, 4000),
     (F, 3, 2000, 101, 3000),               # p below nmin: residues hit 0
-    (F, 10**9, 10**9 + 10**5, 101, 5000),  # cfg1 slice
+    (F, 10**9, 10**9 + 10**5, 101, 5000),  # slice of the -! n [101,5000) p [1e9,1.001e9) job
     (Cm, 10**12, 10**12 + 2 * 10**5, 101, 3000),
     (P, (1 << 62) - 10**5, 1 << 62, 101, 5000),
     (F | Cm, TWO64 - 10**5, TWO64 - 1, 101, 2000),
