@@ -476,3 +476,41 @@ def test_generator_lds_segments(orc, dev, lds, job):
         dev.set_tuning()
     ref = orc.run(*job)
     assert (got.checksum, got.primecount, got.factorcount) == (ref.checksum, ref.primecount, ref.factorcount)
+
+
+def _random_jobs(seed, k):
+    """Seeded small jobs over every mode, p ranges from below 114 to just under
+    2^64, and n ranges from 101 up to a few thousand wide."""
+    import random
+    rng = random.Random(seed)
+    jobs = []
+    while len(jobs) < k:
+        modes = rng.choice([F, P, Cm, F | Cm])
+        nmin = 101 + rng.randrange(3000)
+        nmax = nmin + 1 + rng.randrange(2500)
+        if modes == P and all(nmax % q for q in range(2, int(nmax ** 0.5) + 1)):
+            continue                                  # primorial needs a composite nmax
+        where = rng.randrange(4)
+        if where == 0:
+            pmin = 3 + rng.randrange(20000)
+        elif where == 1:
+            pmin = rng.randrange(10**9, 10**13)
+        elif where == 2:
+            pmin = (1 << 62) - rng.randrange(1, 10**9)
+        else:
+            pmin = TWO64 - 1 - rng.randrange(10**5, 10**9)
+        pmax = min(TWO64 - 1, pmin + 1 + rng.randrange(2 * 10**5))
+        jobs.append((modes, pmin, pmax, nmin, nmax))
+    return jobs
+
+
+@pytest.mark.gpu
+def test_random_jobs_match_oracle(orc, dev):
+    """40 seeded random jobs (every mode, p from the small primes to 2^64) give
+    the oracle's checksum, primecount and factor list."""
+    import pfcsieve_amd as PK
+    for job in _random_jobs(20251203, 40):
+        got = PK.run_sieve(*job, device=dev, verify_threads=8)
+        ref = orc.run(*job)
+        want = [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in ref.factors]
+        assert (got.checksum, got.primecount, got.factors) == (ref.checksum, ref.primecount, want), job
