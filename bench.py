@@ -228,8 +228,8 @@ def other_configs(dev):
     # -! n [1e6, 1e6+256) from p = 1e15, one p-segment.  Set-up time is the
     # segment's device time minus its walk launches (generate and check are
     # ~3 ms of it).
-    nchunks, nbits, mont_k = P.setup_table_size(0, 10**6 - 1)
-    products = nchunks + nbits - 1 + 2 * mont_k.bit_length() + 1
+    nchunks, nbits = P.setup_table_size(0, 10**6 - 1)
+    products = nchunks + nbits - 1 + 1           # chunk products, squarings, the final product by R
     j = P.Job(dev, 1, 10**15, 10**15 + 2 * 10**9, 10**6, 10**6 + 256)
     j.run(1)
     s = j.stats()
@@ -237,7 +237,7 @@ def other_configs(dev):
     setup_s = (s.device_ms - s.walk_ms) * 1e-3
     out["setup_nmin_1e6"] = {"workload": "-! n [1e6,1e6+256) p from 1e15, one p-segment: start residues "
                                          "Mont(999999!) per candidate",
-                             "candidates": s.primecount, "montgomery_products_per_candidate": products,
+                             "candidates": s.primecount, "modular_products_per_candidate": products,
                              "setup_s": setup_s, "candidates_per_s": s.primecount / setup_s,
                              "products_per_s": s.primecount * products / setup_s}
     return out

@@ -230,7 +230,6 @@ typedef struct pfc_setup_table {
     const uint64_t* chunks;    /* device */
     const uint32_t* groups;    /* device: nbits+1 offsets into chunks, group b = bit b */
     uint32_t nbits;
-    uint64_t mont_k;           /* 1 - E, E = the table's Montgomery exponent (see pfc_build_setup_table) */
 } pfc_setup_table;
 
 /* (b) Per candidate: reciprocal, Montgomery one, and start residues
@@ -260,12 +259,11 @@ int pfc_stage_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const ui
                     pfc_ctrl* d_ctrl, uint32_t cap, void* stream);
 
 /* Builds the setup table for X(N) on the host: kind 0 = N!, 1 = N#, 2 = N!/N#.
- * Writes up to chunk_cap chunks and nbits+1 group offsets (group_cap >= 33).
- * Horner's rule over the groups (high bit first) in Montgomery arithmetic
- * without converting the chunks leaves X * 2^(64E); *mont_k = 1 - E. */
+ * Writes up to chunk_cap chunks and nbits+1 group offsets (group_cap >= 33):
+ * X = prod over b of (prod of chunks[groups[b] .. groups[b+1]))^(2^b), which
+ * the set-up evaluates by Horner's rule over b, high bit first. */
 int pfc_build_setup_table(uint32_t kind, uint32_t N, uint64_t* chunks, size_t chunk_cap,
-                          size_t* nchunks, uint32_t* groups, size_t group_cap, uint32_t* nbits,
-                          uint64_t* mont_k);
+                          size_t* nchunks, uint32_t* groups, size_t group_cap, uint32_t* nbits);
 
 /* Start-table self-check (verifyslow.cl, verify.cl, verifyresult.cl;
  * cl_sieve.cpp:917-970, 1079-1134, 1246-1304): X(N) of a table of `kind`

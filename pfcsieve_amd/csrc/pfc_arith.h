@@ -80,9 +80,9 @@ PFC_HD uint64_t div3by2_rem(uint32_t u2, uint32_t u1, uint32_t u0, const Modulus
     return r;
 }
 
-// (a * n) mod d for a < d and any 32-bit n, without the final (rare)
-// correction: returns x with x == (a*n mod d) or x == (a*n mod d) + d, and
-// sets *fix when the caller must subtract d (x >= d).  The walk kernel uses
+// Remainder of <u2,u1,u0> by d, given hi = <u2,u1> < d, without the final
+// (rare) correction: returns x with x == r or x == r + d for the remainder r,
+// and sets *fix when the caller must subtract d (x >= d).  The kernels use
 // this form so that the rare correction becomes a wavefront-uniform branch.
 //
 // The 3-by-2 step forms r = <u1,u0> - (q1 + 1) * d (mod 2^64) with per-modulus
@@ -93,15 +93,13 @@ PFC_HD uint64_t div3by2_rem(uint32_t u2, uint32_t u1, uint32_t u0, const Modulus
 // folded into k1.  u1 + q1 * k1 is the low word of one multiply-add whose
 // 64-bit addend is <u2,u1> as computed, and the two remaining terms are one
 // 64-bit add and one multiply-add.
-PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t k1, uint32_t k0, uint64_t nd, uint64_t d,
-                               uint32_t v, bool* fix) {
-    const uint64_t lo = (uint64_t)(uint32_t)a * n;
-    const uint64_t hi = (uint64_t)(uint32_t)(a >> 32) * n + (lo >> 32);   // <u2,u1>
+PFC_HD uint64_t rem3by2_nofix(uint64_t hi, uint32_t u0, uint32_t k1, uint32_t k0, uint64_t nd, uint64_t d,
+                              uint32_t v, bool* fix) {
     const uint32_t u2 = (uint32_t)(hi >> 32);
     const uint64_t qq = (uint64_t)v * u2 + hi;
     const uint32_t q1 = (uint32_t)(qq >> 32), q0 = (uint32_t)qq;
     const uint32_t w = (uint32_t)((uint64_t)q1 * k1 + hi);              // u1 - q1 * (d1 + 1)
-    const uint64_t pair = (((uint64_t)w) << 32) | (uint32_t)lo;        // <w, u0>
+    const uint64_t pair = (((uint64_t)w) << 32) | u0;                   // <w, u0>
 #if defined(__HIP_DEVICE_COMPILE__)
     // One 64-bit add on the assembled pair.  Written as an instruction so the
     // optimiser cannot split the pair and reassociate the sum into two adds.
@@ -124,6 +122,15 @@ PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t k1, uint32_t k0,
 #endif
     *fix = x >= d;
     return x;
+}
+
+// (a * n) mod d for a < d and any 32-bit n: the 96-bit product <u2,u1,u0>,
+// then rem3by2_nofix (the same contract: x or x + d, *fix when x >= d).
+PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t k1, uint32_t k0, uint64_t nd, uint64_t d,
+                               uint32_t v, bool* fix) {
+    const uint64_t lo = (uint64_t)(uint32_t)a * n;
+    const uint64_t hi = (uint64_t)(uint32_t)(a >> 32) * n + (lo >> 32);   // <u2,u1>
+    return rem3by2_nofix(hi, (uint32_t)lo, k1, k0, nd, d, v, fix);
 }
 
 // The constants k1, k0 of mulmod32_nofix for d = <d1, d0>.
@@ -152,15 +159,29 @@ PFC_HD uint64_t addmod(uint64_t a, uint64_t b, const Modulus& m) {
 
 // (a * b) mod d for a < d and any 64-bit b (b is a plain integer): the
 // 128-bit product <t3,t2,t1,t0>, then two 3-by-2 steps, <t3,t2,t1> mod d and
-// <r,t0> mod d (both satisfy the <u2,u1> < d precondition because a < d).
+// <r,t0> mod d (both satisfy the <u2,u1> < d precondition because a < d and r
+// is canonical).  On the device the rare final corrections are taken as
+// wavefront-uniform branches.
 PFC_HD uint64_t mulmod64(uint64_t a, uint64_t b, const Modulus& m) {
+    const uint64_t d = mod_d(m), nd = 0ull - d;
+    const uint32_t k1 = walk_k1(m.d1, m.d0), k0 = walk_k0(m.d0);
     const uint64_t a0 = (uint32_t)a, a1 = a >> 32, b0 = (uint32_t)b, b1 = b >> 32;
     const uint64_t p00 = a0 * b0;
     const uint64_t p01 = a0 * b1 + (p00 >> 32);
     const uint64_t p10 = a1 * b0 + (uint32_t)p01;
     const uint64_t p11 = a1 * b1 + (p01 >> 32) + (p10 >> 32);
-    const uint64_t r = div3by2_rem((uint32_t)(p11 >> 32), (uint32_t)p11, (uint32_t)p10, m);
-    return div3by2_rem((uint32_t)(r >> 32), (uint32_t)r, (uint32_t)p00, m);
+    bool f1, f2;
+    uint64_t r = rem3by2_nofix(p11, (uint32_t)p10, k1, k0, nd, d, m.v, &f1);
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (__builtin_expect(__ballot(f1) != 0, 0))
+#endif
+        if (r >= d) r -= d;
+    r = rem3by2_nofix(r, (uint32_t)p00, k1, k0, nd, d, m.v, &f2);
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (__builtin_expect(__ballot(f2) != 0, 0))
+#endif
+        if (r >= d) r -= d;
+    return r;
 }
 
 // Square of the residue a' = x << s: (x*x mod p) << s.
@@ -291,72 +312,22 @@ PFC_HD bool is_prime_u32(uint32_t n) {
     return true;
 }
 
-// ---------------------------------------------------------------- Montgomery (start residues)
-
-// p^-1 mod 2^64 for odd p (Newton: each step doubles the correct low bits;
-// x = p is right to 3 bits because p*p == 1 mod 8).
-PFC_HD uint64_t inv64(uint64_t p) {
-    uint64_t x = p;
-    for (int i = 0; i < 5; ++i) x *= 2 - p * x;
-    return x;
-}
-
-// Montgomery product a*b*2^-64 mod p, canonical in [0, p) when a*b < p*2^64
-// (the reduction of kernels/iterate.cl:44-50, written with 32x32->64 pieces:
-// 4 products for a*b, 3 for m = lo(a*b)*q, 4 for hi(m*p)).
-PFC_HD uint64_t redc_mul(uint64_t a, uint64_t b, uint64_t p, uint64_t q) {
-    const uint64_t a0 = (uint32_t)a, a1 = a >> 32, b0 = (uint32_t)b, b1 = b >> 32;
-    const uint64_t p00 = a0 * b0;
-    const uint64_t p01 = a0 * b1 + (p00 >> 32);
-    const uint64_t p10 = a1 * b0 + (uint32_t)p01;
-    const uint64_t hi = a1 * b1 + (p01 >> 32) + (p10 >> 32);
-    const uint64_t lo = (p10 << 32) | (uint32_t)p00;
-    const uint64_t m = lo * q;
-    const uint64_t mp = mulhi64(m, p);
-    // hi - mp with an explicit borrow chain whose final borrow is the
-    // correction condition (one exec-masked add on the device, no compare).
-    uint32_t br0, br1;
-    const uint32_t rlo = subb32((uint32_t)hi, (uint32_t)mp, 0u, &br0);
-    const uint32_t rhi = subb32((uint32_t)(hi >> 32), (uint32_t)(mp >> 32), br0, &br1);
-    uint64_t r = ((uint64_t)rhi << 32) | rlo;
-#if defined(__HIP_DEVICE_COMPILE__)
-    if (br1) {
-        asm volatile("" ::: );
-        r += p;
-    }
-#else
-    if (br1) r += p;
-#endif
-    return r;
-}
+// ---------------------------------------------------------------- start residues
 
 // Start residue (X * 2^64 mod p) << s of X = prod over groups b (high first)
-// of (prod of chunks in group b)^(2^b), by Horner's rule in Montgomery
-// arithmetic without converting the chunks: every product divides by 2^64,
-// so the accumulator ends as X * 2^(64E) with an exponent E < 1 fixed by the
-// table alone; the host passes k = 1 - E and one product with (R^2)^k
-// computed by powering (R^2 mod p is 1 * R^2 in Montgomery terms, and its
-// k-th power is R^(k+1) = R^(2-E)) restores the Montgomery form X * R.
+// of (prod of chunks in group b)^(2^b), by Horner's rule in the shifted
+// representation (every product by mulmod64, chunks as plain integers), then
+// one product by R mod p for the Montgomery form the walk and checksum use.
 template <class Chunks, class Groups>
-PFC_HD uint64_t start_residue(Chunks chunks, Groups groups, uint32_t nbits, uint64_t k,
-                              const Modulus& m, uint64_t one_shifted) {
-    const uint64_t p = mod_d(m) >> m.s;
-    const uint64_t q = inv64(p);
-    uint64_t acc = 1;
+PFC_HD uint64_t start_residue(Chunks chunks, Groups groups, uint32_t nbits, const Modulus& m,
+                              uint64_t one_shifted) {
+    uint64_t acc = 1ull << m.s;                                     // 1, shifted
     for (int b = (int)nbits - 1; b >= 0; --b) {
-        if (b != (int)nbits - 1) acc = redc_mul(acc, acc, p, q);
+        if (b != (int)nbits - 1) acc = mulmod64(acc, acc >> m.s, m);
         const uint32_t g0 = groups[b], g1 = groups[b + 1];
-        for (uint32_t i = g0; i < g1; ++i) acc = redc_mul(acc, chunks[i], p, q);
+        for (uint32_t i = g0; i < g1; ++i) acc = mulmod64(acc, chunks[i], m);
     }
-    // R^2 mod p from R mod p, then its k-th power by left-to-right squaring.
-    const uint64_t one = one_shifted >> m.s;
-    const uint64_t r2 = mulmod64(one_shifted, one, m) >> m.s;
-    uint64_t f = r2;
-    for (int bit = 62 - (int)clz64(k); bit >= 0; --bit) {
-        f = redc_mul(f, f, p, q);
-        if ((k >> bit) & 1) f = redc_mul(f, r2, p, q);
-    }
-    return redc_mul(acc, f, p, q) << m.s;
+    return mulmod64(acc, one_shifted >> m.s, m);
 }
 
 // ---------------------------------------------------------------- wheel + small-prime sieve

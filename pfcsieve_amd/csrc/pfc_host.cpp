@@ -75,7 +75,6 @@ struct HostTable {
     std::vector<uint64_t> chunks;
     std::vector<uint32_t> groups;  // nbits + 1 offsets
     uint32_t nbits = 0;
-    uint64_t mont_k = 1;           // 1 - E (see pfc_build_setup_table)
     uint32_t kind = 0, N = 0;      // X(N) = N!, N# or N!/N#
 };
 
@@ -116,14 +115,6 @@ HostTable build_table(uint32_t kind, uint32_t N) {
         }
     }
     if (T.nbits) T.groups[T.nbits] = (uint32_t)T.chunks.size();
-    // Montgomery exponent of Horner's rule from acc = 1: a square maps e to
-    // 2e - 1, a product with a plain chunk maps e to e - 1.
-    int64_t e = 0;
-    for (int b = (int)T.nbits - 1; b >= 0; --b) {
-        if (b != (int)T.nbits - 1) e = 2 * e - 1;
-        e -= (int64_t)(T.groups[b + 1] - T.groups[b]);
-    }
-    T.mont_k = (uint64_t)(1 - e);
     return T;
 }
 
@@ -465,7 +456,7 @@ int queue_table_check(pfc_job* j) {
     const uint32_t* gs[2] = {j->d_ga, j->d_gb};
     const int n = two ? 2 : 1;
     for (int i = 0; i < n; ++i) {
-        const pfc_setup_table t{cs[i], gs[i], hs[i]->nbits, hs[i]->mont_k};
+        const pfc_setup_table t{cs[i], gs[i], hs[i]->nbits};
         uint64_t* work = j->d_verify + (size_t)i * PFC_VERIFY_WORDS;
         int rc = pfc::launch_verify_table(hs[i]->kind, hs[i]->N, &t, work, j->stream);
         if (rc) return rc;
@@ -587,8 +578,8 @@ static int build_graph(pfc_job* j) {
     e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
     int rc = pfc::launch_head_dev(j->d_bounds, j->plan.span, j->d_p, cap, j->d_ctrl, st, j->tuning.lds_segment);
-    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits, j->ta.mont_k};
-    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits, j->tb.mont_k};
+    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits};
+    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits};
     if (!rc)
         rc = pfc::launch_setup(j->d_p, j->d_ctrl, j->d_v, j->d_one, j->d_ra, j->d_rb, &ta,
                                j->d_rb ? &tb : nullptr, cap, st);
@@ -639,8 +630,8 @@ static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
     if (e != hipSuccess) return hip_fail(e, "ctrl clear");
     int rc = pfc::launch_generate(lo, hi, j->d_p, cap, j->d_ctrl, st, j->tuning.lds_segment);
     if (rc) return rc;
-    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits, j->ta.mont_k};
-    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits, j->tb.mont_k};
+    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits};
+    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits};
     rc = pfc::launch_setup(j->d_p, j->d_ctrl, j->d_v, j->d_one, j->d_ra, j->d_rb, &ta,
                            j->d_rb ? &tb : nullptr, cap, st);
     if (rc) return rc;
@@ -839,8 +830,8 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
     hipError_t e = hipMemsetAsync(j->d_ctrl, 0, sizeof(pfc_ctrl), st);
     if (e != hipSuccess) rc = hip_fail(e, "autotune clear");
     if (!rc) rc = pfc::launch_generate(d.pmin, d.pmin + j->plan.span, j->d_p, cap, j->d_ctrl, st, t.lds_segment);
-    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits, j->ta.mont_k};
-    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits, j->tb.mont_k};
+    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits};
+    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits};
     if (!rc)
         rc = pfc::launch_setup(j->d_p, j->d_ctrl, j->d_v, j->d_one, j->d_ra, j->d_rb, &ta,
                                j->d_rb ? &tb : nullptr, cap, st);
@@ -977,8 +968,6 @@ pfc_batch_job batch_view(pfc_job* j) {
     b.fcap = (uint32_t)j->fcap;
     b.nba = j->ta.nbits;
     b.nbb = j->tb.nbits;
-    b.ka = j->ta.mont_k;
-    b.kb = j->tb.mont_k;
     b.ca = j->d_ca;
     b.ga = j->d_ga;
     b.cb = j->d_cb;
@@ -1395,12 +1384,11 @@ extern "C" int pfc_parse_uint64(const char* s, uint64_t lo, uint64_t hi, uint64_
 
 extern "C" int pfc_build_setup_table(uint32_t kind, uint32_t N, uint64_t* chunks, size_t chunk_cap,
                                      size_t* nchunks, uint32_t* groups, size_t group_cap,
-                                     uint32_t* nbits, uint64_t* mont_k) {
+                                     uint32_t* nbits) {
     if (kind > 2 || !nchunks || !nbits) return set_error(PFC_ERR_INVALID, "build_setup_table: bad args");
     HostTable T = build_table(kind, N);
     *nchunks = T.chunks.size();
     *nbits = T.nbits;
-    if (mont_k) *mont_k = T.mont_k;
     if (chunks) memcpy(chunks, T.chunks.data(), std::min(chunk_cap, T.chunks.size()) * 8);
     if (groups) memcpy(groups, T.groups.data(), std::min(group_cap, T.groups.size()) * 4);
     return PFC_OK;

@@ -20,7 +20,6 @@ struct pfc_batch_job {
     pfc_ctrl* ctrl;
     pfc_factor* fac;
     uint32_t fcap, nba, nbb, pad;
-    uint64_t ka, kb;  // Montgomery exponents of tables A and B
     const uint64_t* ca;
     const uint32_t* ga;
     const uint64_t* cb;

@@ -158,9 +158,8 @@ typedef uint32_t U32x16 __attribute__((ext_vector_type(16)));
 typedef const U32x16 __attribute__((address_space(4)))* ConstU32x16;
 
 __device__ __forceinline__ uint64_t horner_start(const uint64_t* chunks, const uint32_t* groups,
-                                                 uint32_t nbits, uint64_t k, const Modulus& m,
-                                                 uint64_t one) {
-    return pfc::start_residue((ConstU64)chunks, (ConstU32)groups, nbits, k, m, one);
+                                                 uint32_t nbits, const Modulus& m, uint64_t one) {
+    return pfc::start_residue((ConstU64)chunks, (ConstU32)groups, nbits, m, one);
 }
 
 __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
@@ -169,9 +168,8 @@ __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
                                                uint64_t* __restrict__ RA, uint64_t* __restrict__ RB,
                                                const uint64_t* __restrict__ ca,
                                                const uint32_t* __restrict__ ga, uint32_t nba,
-                                               uint64_t ka, const uint64_t* __restrict__ cb,
-                                               const uint32_t* __restrict__ gb, uint32_t nbb,
-                                               uint64_t kb, uint32_t cap) {
+                                               const uint64_t* __restrict__ cb,
+                                               const uint32_t* __restrict__ gb, uint32_t nbb, uint32_t cap) {
     uint32_t n = ctrl->ncand;
     if (n > cap) n = cap;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -180,8 +178,8 @@ __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
         const uint64_t one = pfc::mont_one(m);
         V[i] = m.v;
         ONE[i] = one;
-        RA[i] = horner_start(ca, ga, nba, ka, m, one);
-        if (RB) RB[i] = horner_start(cb, gb, nbb, kb, m, one);
+        RA[i] = horner_start(ca, ga, nba, m, one);
+        if (RB) RB[i] = horner_start(cb, gb, nbb, m, one);
     }
 }
 
@@ -218,14 +216,14 @@ __global__ __launch_bounds__(256) void k_verify_slow(uint32_t kind, uint32_t N, 
 
 __global__ __launch_bounds__(64) void k_verify_finish(const uint64_t* __restrict__ chunks,
                                                       const uint32_t* __restrict__ groups, uint32_t nbits,
-                                                      uint64_t k, uint64_t* __restrict__ work) {
+                                                      uint64_t* __restrict__ work) {
     if (threadIdx.x != 0) return;
     const Modulus m = pfc::make_modulus(kTestPrime);
     uint64_t x = 1;
     for (uint32_t i = 0; i < PFC_VERIFY_BLOCKS; ++i) x = pfc::mulmod64(x, work[2 + i], m);
     const uint64_t one = pfc::mont_one(m);
     work[0] = pfc::mulmod64(one, x, m);                  // X * R mod p
-    work[1] = horner_start(chunks, groups, nbits, k, m, one);
+    work[1] = horner_start(chunks, groups, nbits, m, one);
 }
 
 // ------------------------------------------------------------------ walk
@@ -551,8 +549,8 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
             const Modulus m = pfc::make_modulus(p);
             const uint64_t one = pfc::mont_one(m);
             lanes_from_state<K>(L, p, m.v, one, k);
-            ra[k] = horner_start(J.ca, J.ga, J.nba, J.ka, m, one);
-            rb[k] = two ? horner_start(J.cb, J.gb, J.nbb, J.kb, m, one) : 0;
+            ra[k] = horner_start(J.ca, J.ga, J.nba, m, one);
+            rb[k] = two ? horner_start(J.cb, J.gb, J.nbb, m, one) : 0;
         } else {
             lanes_from_state<K>(L, p, J.V[src], J.ONE[src], k);
             ra[k] = J.RA[src];
@@ -725,8 +723,8 @@ int launch_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uin
     if (blocks == 0) blocks = 1;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, st, d_p, d_ctrl, d_v, d_one, d_ra, d_rb,
-                       a->chunks, a->groups, a->nbits, a->mont_k, b ? b->chunks : nullptr,
-                       b ? b->groups : nullptr, b ? b->nbits : 0u, b ? b->mont_k : 1ull, cap);
+                       a->chunks, a->groups, a->nbits, b ? b->chunks : nullptr,
+                       b ? b->groups : nullptr, b ? b->nbits : 0u, cap);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_setup launch");
 }
@@ -806,7 +804,7 @@ int launch_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uin
     hipLaunchKernelGGL(k_verify_slow, dim3(PFC_VERIFY_BLOCKS), dim3(256), 0, st, kind, N, d_work);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "k_verify_slow launch");
-    hipLaunchKernelGGL(k_verify_finish, dim3(1), dim3(64), 0, st, t->chunks, t->groups, t->nbits, t->mont_k,
+    hipLaunchKernelGGL(k_verify_finish, dim3(1), dim3(64), 0, st, t->chunks, t->groups, t->nbits,
                        d_work);
     e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_verify_finish launch");

@@ -186,11 +186,11 @@ def footprint(dev: Optional[Device], modes, pmin, pmax, nmin, nmax) -> int:
 
 
 def setup_table_size(kind: int, N: int):
-    """(chunks, exponent bits, Montgomery fix exponent k) of the start-residue
-    table for X(N) with kind 0 = N!, 1 = N#, 2 = N!/N# (host only)."""
-    nch, nb, mk = C.c_size_t(), C.c_uint32(), C.c_uint64()
-    check(lib().pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb), C.byref(mk)))
-    return nch.value, nb.value, mk.value
+    """(chunks, exponent bits) of the start-residue table for X(N) with
+    kind 0 = N!, 1 = N#, 2 = N!/N# (host only)."""
+    nch, nb = C.c_size_t(), C.c_uint32()
+    check(lib().pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb)))
+    return nch.value, nb.value
 
 
 def format_factor(f: Factor) -> str:

@@ -20,12 +20,10 @@ uint64_t t_mont_one(uint64_t p) { return pfc::mont_one(pfc::make_modulus(p)); }
 int t_modulus_ok(uint64_t p) { return pfc::modulus_ok(p, pfc::make_modulus(p)); }
 int t_strong_prp2(uint64_t n) { return pfc::strong_prp2(n); }
 int t_is_prime_u32(uint32_t n) { return pfc::is_prime_u32(n); }
-uint64_t t_start_residue(const uint64_t* chunks, const uint32_t* groups, uint32_t nbits, uint64_t k,
-                         uint64_t p) {
+uint64_t t_start_residue(const uint64_t* chunks, const uint32_t* groups, uint32_t nbits, uint64_t p) {
     const pfc::Modulus m = pfc::make_modulus(p);
-    return pfc::start_residue(chunks, groups, nbits, k, m, pfc::mont_one(m));
+    return pfc::start_residue(chunks, groups, nbits, m, pfc::mont_one(m));
 }
-uint64_t t_redc(uint64_t a, uint64_t b, uint64_t p) { return pfc::redc_mul(a, b, p, pfc::inv64(p)); }
 // Candidates of [lo, hi) (lo >= 114) exactly as k_generate's lanes compute them.
 size_t t_generate(uint64_t lo, uint64_t hi, uint64_t* out, size_t cap) {
     size_t n = 0;

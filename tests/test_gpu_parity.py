@@ -167,12 +167,11 @@ def _table(dev, kind, N, corrupt=False):
     import pfcsieve_amd._native as Nt
     from pfcsieve_amd.sieve import DeviceBuffer
     L = Nt.lib()
-    nch, nb, mk = C.c_size_t(), C.c_uint32(), C.c_uint64()
-    Nt.check(L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb), C.byref(mk)))
+    nch, nb = C.c_size_t(), C.c_uint32()
+    Nt.check(L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb)))
     ch = (C.c_uint64 * max(nch.value, 1))()
     gr = (C.c_uint32 * (nb.value + 1))()
-    Nt.check(L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb),
-                                     C.byref(mk)))
+    Nt.check(L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb)))
     if corrupt:
         assert nch.value > 0
         ch[0] = ch[0] + 2 if ch[0] < 2**63 else ch[0] - 2
@@ -180,7 +179,7 @@ def _table(dev, kind, N, corrupt=False):
     dc.upload(bytes(ch))
     dg = DeviceBuffer(dev, 4 * (nb.value + 1))
     dg.upload(bytes(gr))
-    return Nt.SetupTable(dc.ptr.value, dg.ptr.value, nb.value, mk.value), (dc, dg)
+    return Nt.SetupTable(dc.ptr.value, dg.ptr.value, nb.value), (dc, dg)
 
 
 @pytest.mark.parametrize("modes,nmin,nmax", [(F, 2000, 2600), (Cm, 2000, 2600), (F | Cm, 2000, 2600)])

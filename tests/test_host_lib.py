@@ -119,12 +119,11 @@ def test_verify_factor(L):
 @pytest.mark.parametrize("kind,N", [(0, 100), (0, 4999), (0, 100000), (1, 100), (1, 99999),
                                     (2, 100), (2, 77777)])
 def test_setup_table_reconstructs_start_value(L, kind, N):
-    nch, nb, mk = C.c_size_t(), C.c_uint32(), C.c_uint64()
-    L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb), C.byref(mk))
+    nch, nb = C.c_size_t(), C.c_uint32()
+    L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb))
     ch = (C.c_uint64 * nch.value)()
     gr = (C.c_uint32 * (nb.value + 1))()
-    assert L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb),
-                                   C.byref(mk)) == 0
+    assert L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb)) == 0
     x = 1
     for b in range(nb.value - 1, -1, -1):      # Horner over exponent bits, high first
         x = x * x
@@ -161,10 +160,8 @@ def A():
     L.t_is_prime_u32.argtypes = [u32]
     L.t_generate.argtypes = [u64, u64, C.POINTER(u64), C.c_size_t]
     L.t_generate.restype = C.c_size_t
-    L.t_start_residue.argtypes = [C.POINTER(u64), C.POINTER(u32), u32, u64, u64]
+    L.t_start_residue.argtypes = [C.POINTER(u64), C.POINTER(u32), u32, u64]
     L.t_start_residue.restype = u64
-    L.t_redc.argtypes = [u64, u64, u64]
-    L.t_redc.restype = u64
     return L
 
 
@@ -245,27 +242,19 @@ def test_generate_window_logic_matches_oracle(A, orc, lo, hi):
     assert list(out[:n]) == [int(x) for x in orc.candidates(lo, hi)]
 
 
-def test_redc_against_bigint(A):
-    rng = random.Random(11)
-    for p in _primes_mix(rng, 2000):
-        for _ in range(3):
-            a, b = rng.randrange(p), rng.randrange(R)
-            assert A.t_redc(a, b, p) == a * b * pow(R, -1, p) % p
-
-
 @pytest.mark.parametrize("kind,N", [(0, 100), (0, 4999), (0, 99999), (1, 100), (1, 77777), (2, 100), (2, 65432)])
 def test_start_residue_against_bigint(L, A, kind, N):
-    """Horner in Montgomery arithmetic with the exponent correction gives
-    (X * 2^64 mod p) << s for X = N!, N#, N!/N#."""
-    nch, nb, mk = C.c_size_t(), C.c_uint32(), C.c_uint64()
-    L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb), C.byref(mk))
+    """Horner's rule over the table in the shifted representation, then the
+    product by R, gives (X * 2^64 mod p) << s for X = N!, N#, N!/N#."""
+    nch, nb = C.c_size_t(), C.c_uint32()
+    L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb))
     ch = (C.c_uint64 * max(nch.value, 1))()
     gr = (C.c_uint32 * (nb.value + 1))()
-    L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb), C.byref(mk))
+    L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb))
     prim = sympy.primorial(N, nth=False)
     X = [math.factorial(N), prim, math.factorial(N) // prim][kind]
     rng = random.Random(kind * 1000 + N)
     for p in _primes_mix(rng, 40) + [101, 103, 10007, N + 1 if (N + 1) % 2 else N + 2]:
         s = 64 - p.bit_length()
-        got = A.t_start_residue(ch, gr, nb.value, mk.value, p)
+        got = A.t_start_residue(ch, gr, nb.value, p)
         assert got == (X % p) * R % p << s, (kind, N, p)
