@@ -112,3 +112,13 @@ def test_corrupt_checkpoint_ignored(tmp_path, orc):
     assert "Invalid checkpoint file stateA.ckp" in r.stdout and "Cannot parse stateB.ckp" in r.stdout
     want, _ = _oracle_file(orc, *job)
     assert (tmp_path / "factors.txt").read_text() == want
+
+
+@pytest.mark.gpu
+def test_cli_self_test(tmp_path):
+    """`pfcsieve -s`: the reference's 16-range self-test (cl_sieve.cpp:1777-2095)
+    run by the CLI on the GPU; every case must match its golden triple."""
+    r = run(["-s", "-v", "8"], tmp_path, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert r.stdout.count("passed.") == 16
+    assert "All test cases completed successfully!" in r.stdout
