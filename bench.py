@@ -3,14 +3,14 @@
 Workload (default): plain factorial + compositorial sieve, -! -c,
 n in [101, 1,000,000), p from 1e12 (the 2e9-wide p-range of the headline
 configuration, processed one p-segment at a time).  One "step" is one
-p-segment of about 2^22 candidates run through generate -> setup -> walk ->
+p-segment of about 2^24 candidates run through generate -> setup -> walk ->
 check, with its factor records drained; inputs are generated on the device.
 
 Prints ONE JSON line.  `value` is p*n residue updates per second over the
 timed steps (updates = candidates x multiplier steps, F and C walks counted
 separately).  `roofline` prices the updates at 19 INT32 multiplies each (the
-reference's 64-bit Montgomery product; see the roofline section of the
-design notes) against the measured INT32 multiply lane-op rate of the device.
+reference's 64-bit Montgomery product, 11 32x32->64 partial products) against
+the measured INT32 multiply lane-op rate of the device.
 `cpu_baseline` times the CPU oracle (OpenMP, all host cores) on a bounded
 p-sample of the same workload.
 """
@@ -265,11 +265,12 @@ def shape_sweep(dev, modes, pmin, nmin, nmax):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--seg-candidates", type=int, default=1 << 22)
+    ap.add_argument("--seg-candidates", type=int, default=1 << 24)
     ap.add_argument("--ppl", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--walk-chunk", type=int, default=0, help="multiplier steps per walk launch (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-width", type=int, default=1_000_000)
     ap.add_argument("--mix-units", type=int, default=24,
@@ -299,7 +300,8 @@ def main():
         dev.probe_copy(args.copy_probe)
     rates = dev.probe_mul_rates()
     p_mul = max(rates.values())
-    dev.set_tuning(seg_candidates=args.seg_candidates, primes_per_lane=args.ppl, block_size=args.block)
+    dev.set_tuning(seg_candidates=args.seg_candidates, primes_per_lane=args.ppl, block_size=args.block,
+                   walk_chunk=args.walk_chunk)
     if args.autotune:
         dev.autotune(modes, pmin, pmin + 2 * 10**9, nmin, nmax)
     total = args.steps + args.warmup
@@ -350,7 +352,8 @@ def main():
         line["other_configs"] = other_configs(dev)
     if args.sweep:
         line["shape_sweep"] = shape_sweep(dev, modes, pmin, nmin, nmax)
-        dev.set_tuning(seg_candidates=args.seg_candidates, primes_per_lane=args.ppl, block_size=args.block)
+        dev.set_tuning(seg_candidates=args.seg_candidates, primes_per_lane=args.ppl, block_size=args.block,
+                       walk_chunk=args.walk_chunk)
     if args.mix_units:
         from pfcsieve_amd import workload as W
         units = W.mix(256, seed=12345)[: args.mix_units]

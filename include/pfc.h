@@ -70,8 +70,8 @@ typedef struct pfc_job_desc {
 typedef struct pfc_tuning {
     uint32_t primes_per_lane;  /* independent residue chains per lane: 1, 2, 4 */
     uint32_t block_size;       /* walk workgroup size: 64, 128, 256 */
-    uint32_t walk_chunk;       /* multiplier steps per walk launch */
-    uint32_t seg_candidates;   /* target candidates per p-segment */
+    uint32_t walk_chunk;       /* multiplier steps per walk launch (default 2^18) */
+    uint32_t seg_candidates;   /* target candidates per p-segment (default 2^24) */
     uint32_t use_graph;        /* 1: capture a job's segment pipeline (clear, generate, setup,
                                   walk chunks, check) once as a HIP graph and replay it per segment */
     uint32_t skip_table_check; /* 1: do not self-check the start tables at job creation

@@ -151,8 +151,12 @@ static pfc_tuning default_tuning() {
     pfc_tuning t;
     t.primes_per_lane = 1;   // fastest walk shape measured on MI355X (pfc_autotune)
     t.block_size = 256;
-    t.walk_chunk = 1u << 16;
-    t.seg_candidates = 1u << 22;
+    // 2^24-candidate segments give each walk launch 32 rounds of wavefronts
+    // per wave slot, so the ramp-up and drain at the ends of a launch cost
+    // ~1.5% (2^22: ~6%); 2^18 steps per launch keep launches few (measured on
+    // the headline: 3.40e12 against 3.25e12 updates/s at 2^22 and 2^16).
+    t.walk_chunk = 1u << 18;
+    t.seg_candidates = 1u << 24;
     t.use_graph = 0;
     t.skip_table_check = 0;
     t.lds_segment = 15360;   // 256 threads x one 60-integer window
