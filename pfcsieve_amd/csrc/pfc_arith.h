@@ -10,9 +10,9 @@
 // Reduction is the 3-by-2 division with a precomputed reciprocal
 //   v = floor((2^96 - 1) / d) - 2^32
 // (N. Moller and T. Granlund, "Improved division by invariant integers",
-// IEEE Trans. Computers 60(2), 2011, Algorithm 5).  A 64x32-bit modular
-// product costs four 32x32->64 multiplies plus one 32-bit low multiply, which
-// maps onto v_mad_u64_u32 / v_mul_lo_u32 on CDNA.
+// IEEE Trans. Computers 60(2), 2011, Algorithm 5), with the remainder formed
+// from per-modulus constants (rem3by2_nofix).  A 64x32-bit modular product
+// costs five 32x32+64 multiply-adds (v_mad_u64_u32 on CDNA).
 //
 // The reference (kernels/iterate.cl:44-57) multiplies 64-bit Montgomery forms
 // with R = 2^64.  Its checksum sums Montgomery forms X*R mod p, so residues
@@ -32,20 +32,6 @@
 
 namespace pfc {
 
-// 32-bit subtract with borrow in/out.
-PFC_HD uint32_t subb32(uint32_t a, uint32_t b, uint32_t bin, uint32_t* bout) {
-#if defined(__clang__)
-    unsigned int o;
-    const uint32_t r = __builtin_subc(a, b, bin, &o);
-    *bout = o;
-    return r;
-#else
-    const uint64_t r = (uint64_t)a - b - bin;
-    *bout = (uint32_t)(r >> 63);
-    return (uint32_t)r;
-#endif
-}
-
 struct Modulus {
     uint32_t d1, d0;  // d = p << s, high and low words (d1 >= 2^31)
     uint32_t v;       // 3-by-2 reciprocal of d
@@ -62,22 +48,6 @@ PFC_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
     uint64_t p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0, p11 = a1 * b1;
     uint64_t mid = (p00 >> 32) + (uint32_t)p01 + (uint32_t)p10;
     return p11 + (p01 >> 32) + (p10 >> 32) + (mid >> 32);
-}
-
-// Remainder of the 3-word number <u2,u1,u0> by d, given <u2,u1> < d.
-// <q1,q0> estimates the quotient; r = <u1 - q1*d1, u0> - d0*q1 - d is then the
-// remainder for quotient q1 + 1, taken mod 2^64, and at most two corrections
-// by d follow (Algorithm 5 of Moller-Granlund).  d0*q1 + d is formed in one
-// 32x32+64 multiply-add; its carry out of 64 bits cancels mod 2^64.
-PFC_HD uint64_t div3by2_rem(uint32_t u2, uint32_t u1, uint32_t u0, const Modulus& m) {
-    const uint64_t qq = (uint64_t)m.v * u2 + (((uint64_t)u2 << 32) | u1);
-    const uint32_t q1 = (uint32_t)(qq >> 32), q0 = (uint32_t)qq;
-    const uint32_t r1 = u1 - q1 * m.d1;
-    const uint64_t d = mod_d(m);
-    uint64_t r = ((((uint64_t)r1) << 32) | u0) - ((uint64_t)m.d0 * q1 + d);
-    if ((uint32_t)(r >> 32) >= q0) r += d;
-    if (r >= d) r -= d;
-    return r;
 }
 
 // Remainder of <u2,u1,u0> by d, given hi = <u2,u1> < d, without the final
@@ -143,11 +113,6 @@ PFC_HD uint64_t mulmod32(uint64_t a, uint32_t n, const Modulus& m) {
     const uint64_t d = mod_d(m);
     const uint64_t x = mulmod32_nofix(a, n, walk_k1(m.d1, m.d0), walk_k0(m.d0), 0ull - d, d, m.v, &fix);
     return fix ? x - d : x;
-}
-
-// (a * 2^32) mod d for a < d.
-PFC_HD uint64_t shl32mod(uint64_t a, const Modulus& m) {
-    return div3by2_rem((uint32_t)(a >> 32), (uint32_t)a, 0u, m);
 }
 
 // (a + b) mod d for a, b < d (d >= 2^63, so a + b may carry out of 64 bits).
