@@ -55,7 +55,11 @@ PFC_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
 // and sets *fix when the caller must subtract d (x >= d).  The kernels use
 // this form so that the rare correction becomes a wavefront-uniform branch.
 //
-// The 3-by-2 step forms r = <u1,u0> - (q1 + 1) * d (mod 2^64) with per-modulus
+// Algorithm 5 of Moller-Granlund: <q1,q0> = v*u2 + <u2,u1> estimates the
+// quotient; r = <u1,u0> - (q1 + 1)*d (mod 2^64) is the remainder for quotient
+// q1 + 1, d is added back when the high word of r is >= q0 (the quotient was
+// one too large), and a last, unlikely, subtraction of d covers a quotient
+// one too small.  This step forms r = <u1,u0> - (q1 + 1)*d with per-modulus
 // constants instead of a borrow chain: with k1 = -(d1 + 1) and k0 = 2^32 - d0
 // (k1 = -d1 and k0 = 0 when d0 == 0) and nd = 2^64 - d,
 //   r = q1 * k0 + (<u1 + q1 * k1, u0> + nd)      (mod 2^64),
