@@ -671,9 +671,11 @@ static int drain_segment(pfc_job* j, bool* overflow, uint64_t launches, bool tim
     }
     if (c.flags & 2u) return set_error(PFC_ERR_VALIDATION, "error: gpu validation failure");
     if (c.nfactor > j->fcap) {
-        char msg[128];
-        snprintf(msg, sizeof(msg), "Error: number of results (%u) overflowed array.", c.nfactor);
-        return set_error(PFC_ERR_FACTOR_OVERFLOW, msg);
+        // More factor records than the buffer holds (the reference exits here,
+        // cl_sieve.cpp:484-487, but its segments are far smaller): the segment
+        // is redone in halves, like a candidate overflow.
+        *overflow = true;
+        return PFC_OK;
     }
     if (c.nfactor) {
         const size_t old = j->factors.size();
@@ -708,8 +710,16 @@ extern "C" int pfc_job_run(pfc_job* j, uint64_t max_segments) {
         bool overflow = false;
         int rc = run_segment(j, j->pos, hi, &overflow);
         if (rc) return rc;
-        if (overflow) {  // more candidates than the bound: split the segment
-            if (span <= 1) return set_error(PFC_ERR_CAPACITY, "candidate buffer overflow");
+        if (overflow) {  // more candidates or factor records than the buffers hold: split the segment
+            if (span <= 1) {
+                const pfc_ctrl& c = *j->h_ctrl;
+                if (c.nfactor > j->fcap) {
+                    char msg[128];
+                    snprintf(msg, sizeof(msg), "Error: number of results (%u) overflowed array.", c.nfactor);
+                    return set_error(PFC_ERR_FACTOR_OVERFLOW, msg);
+                }
+                return set_error(PFC_ERR_CAPACITY, "candidate buffer overflow");
+            }
             span /= 2;
             continue;
         }
@@ -997,11 +1007,18 @@ int ensure_device(T** p, size_t* cap, size_t need) {
 }
 
 // Drains a finished segment into the job (factor records, sums, position).
-int finish_segment(pfc_job* j, const pfc_ctrl& c, uint64_t seg_hi) {
+// *redo: the segment overflowed the factor buffer and must be walked again
+// in halves (nothing of it is kept).
+int finish_segment(pfc_job* j, const pfc_ctrl& c, uint64_t seg_hi, uint64_t span, bool* redo) {
+    *redo = false;
     int rc = check_tables(j);
     if (rc) return rc;
     if (c.flags & 2u) return set_error(PFC_ERR_VALIDATION, "error: gpu validation failure");
     if (c.nfactor > j->fcap) {
+        if (span > 1) {
+            *redo = true;
+            return PFC_OK;
+        }
         char msg[128];
         snprintf(msg, sizeof(msg), "Error: number of results (%u) overflowed array.", c.nfactor);
         return set_error(PFC_ERR_FACTOR_OVERFLOW, msg);
@@ -1166,8 +1183,10 @@ extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** d
     for (size_t i = 0; i < ns; ++i) {
         pfc_batcher::Slot s = b->slots[i];
         if (finishing[i]) {
-            rc = finish_segment(s.job, b->h_ctrl[i], s.seg_hi);
+            bool redo = false;
+            rc = finish_segment(s.job, b->h_ctrl[i], s.seg_hi, s.span, &redo);
             if (rc) return rc;
+            if (redo) s.span /= 2;
             s.stage = s.job->pos >= s.job->desc.pmax ? 2 : 0;
         }
         if (s.stage == 2) {

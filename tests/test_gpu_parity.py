@@ -513,3 +513,32 @@ def test_random_jobs_match_oracle(orc, dev):
         ref = orc.run(*job)
         want = [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in ref.factors]
         assert (got.checksum, got.primecount, got.factors) == (ref.checksum, ref.primecount, want), job
+
+
+@pytest.mark.gpu
+def test_factor_buffer_overflow_splits_segments(orc, dev):
+    """A segment whose factor records exceed the buffer is walked again in
+    halves (plain and batched), giving the oracle's results; a single prime
+    that alone overflows it fails with PFC_ERR_FACTOR_OVERFLOW."""
+    import pfcsieve_amd as PK
+    job = (F, 101, 30000, 101, 3000)
+    ref = orc.run(*job)
+    assert ref.raw_factors > 1000
+    got = PK.run_sieve(*job, device=dev, verify_threads=8, factor_capacity=300)
+    want = [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in ref.factors]
+    assert (got.checksum, got.primecount, got.factors) == (ref.checksum, ref.primecount, want)
+    assert got.segments > 4
+    j = PK.Job(dev, *job, factor_capacity=300)
+    b = PK.Batcher(dev)
+    try:
+        b.add(j)
+        while b.pending():
+            b.round(0)
+        g = PK.job_result(j, verify_threads=8)
+        assert (g.checksum, g.primecount, g.factors) == (ref.checksum, ref.primecount, want)
+    finally:
+        b.destroy()
+        j.destroy()
+    with pytest.raises(PK.PfcError) as e:
+        PK.run_sieve(F, 103, 104, 101, 103, device=dev, factor_capacity=1)   # 101!-1 and 102!+1
+    assert e.value.code == -4
