@@ -232,9 +232,11 @@ typedef struct pfc_setup_table {
     uint32_t nbits;
 } pfc_setup_table;
 
-/* (b) Per candidate: reciprocal, Montgomery one, and start residues
- * Mont(X(nmin-1)) for table A (F, P or C) and table B (C in F+C mode)
- * (setup.cl:49-288).  Residues are stored shifted left by clz64(p). */
+/* (b) Per candidate: reciprocal, Montgomery one (2^64 mod p) and start
+ * residues X(nmin-1) mod p for table A (F, P or C) and table B (C in F+C
+ * mode) (setup.cl:49-288).  Residues are plain (not Montgomery forms) and
+ * stored shifted left by clz64(p); the checksum fold forms the Montgomery
+ * terms the reference sums. */
 int pfc_stage_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uint64_t* d_one,
                     uint64_t* d_res_a, uint64_t* d_res_b, const pfc_setup_table* a,
                     const pfc_setup_table* b, uint32_t cap, void* stream);
@@ -248,9 +250,8 @@ int pfc_stage_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, 
  * Records (p, -+n, type) when a residue becomes +-1, up to fcap records. */
 int pfc_stage_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list,
                    const uint32_t* d_composite, uint32_t nbase, const uint64_t* d_p,
-                   const uint32_t* d_v, const uint64_t* d_one, uint64_t* d_res_a,
-                   uint64_t* d_res_b, pfc_ctrl* d_ctrl, pfc_factor* d_fac, uint32_t fcap,
-                   uint32_t cap, void* stream);
+                   const uint32_t* d_v, uint64_t* d_res_a, uint64_t* d_res_b, pfc_ctrl* d_ctrl,
+                   pfc_factor* d_fac, uint32_t fcap, uint32_t cap, void* stream);
 
 /* (c) Checksum fold (check.cl:45-204): adds every candidate's term to
  * ctrl->checksum and ncand to ctrl->primecount; validates the reciprocals. */

@@ -97,7 +97,7 @@ SIGNATURES = {
     "pfc_stage_generate": (_int, [_u64, _u64, _vp, _u32, _vp, _vp]),
     "pfc_stage_setup": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(SetupTable),
                                C.POINTER(SetupTable), _u32, _vp]),
-    "pfc_stage_walk": (_int, [_u32, _u32, _u32, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+    "pfc_stage_walk": (_int, [_u32, _u32, _u32, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp,
                               _u32, _u32, _vp]),
     "pfc_stage_check": (_int, [_u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
     "pfc_stage_verify_table": (_int, [_u32, _u32, C.POINTER(SetupTable), _vp, _vp]),

@@ -15,10 +15,11 @@
 // costs five 32x32+64 multiply-adds (v_mad_u64_u32 on CDNA).
 //
 // The reference (kernels/iterate.cl:44-57) multiplies 64-bit Montgomery forms
-// with R = 2^64.  Its checksum sums Montgomery forms X*R mod p, so residues
-// here carry the same factor R: a walk starts from (X0*R mod p) << s and is
-// multiplied by plain n, which keeps the factor R.  Results are therefore the
-// same canonical integers the reference produces.
+// with R = 2^64 and its checksum sums Montgomery forms X*R mod p.  Here the
+// walk keeps plain residues X << s, so that +1 and -1 are 2^s and d - 2^s,
+// the two ends of the residue range, and the checksum term X*R mod p is
+// formed once per candidate at the end.  Both give the same canonical
+// integers (residues, factor records and checksum terms).
 #ifndef PFC_ARITH_H
 #define PFC_ARITH_H
 
@@ -106,6 +107,14 @@ PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t k1, uint32_t k0,
     const uint64_t hi = (uint64_t)(uint32_t)(a >> 32) * n + (lo >> 32);   // <u2,u1>
     return rem3by2_nofix(hi, (uint32_t)lo, k1, k0, nd, d, v, fix);
 }
+
+// The walk's test for a result x of mulmod32_nofix, with one = 2^s (the
+// shifted 1), c = ~one and thr = d - 2*one - 1:  t = x + c = x - one - 1
+// (mod 2^64) is at least thr exactly when x <= one (t wraps) or x >= d - one.
+// That covers x == 1 and x == -1 (one and d - one) and every unreduced x
+// (x >= d) with one add and one compare; the only other values it passes are
+// the 2^(s+1) - 1 integers x < one or d - one < x < d, a fraction 2/p.
+PFC_HD bool walk_attention(uint64_t x, uint64_t c, uint64_t thr) { return x + c >= thr; }
 
 // The constants k1, k0 of mulmod32_nofix for d = <d1, d0>.
 PFC_HD uint32_t walk_k1(uint32_t d1, uint32_t d0) { return d0 ? ~d1 : 0u - d1; }
@@ -283,20 +292,24 @@ PFC_HD bool is_prime_u32(uint32_t n) {
 
 // ---------------------------------------------------------------- start residues
 
-// Start residue (X * 2^64 mod p) << s of X = prod over groups b (high first)
-// of (prod of chunks in group b)^(2^b), by Horner's rule in the shifted
-// representation (every product by mulmod64, chunks as plain integers), then
-// one product by R mod p for the Montgomery form the walk and checksum use.
+// Start residue (X mod p) << s of X = prod over groups b (high first) of
+// (prod of chunks in group b)^(2^b), by Horner's rule in the shifted
+// representation (every product by mulmod64, chunks as plain integers).
 template <class Chunks, class Groups>
-PFC_HD uint64_t start_residue(Chunks chunks, Groups groups, uint32_t nbits, const Modulus& m,
-                              uint64_t one_shifted) {
+PFC_HD uint64_t start_residue(Chunks chunks, Groups groups, uint32_t nbits, const Modulus& m) {
     uint64_t acc = 1ull << m.s;                                     // 1, shifted
     for (int b = (int)nbits - 1; b >= 0; --b) {
         if (b != (int)nbits - 1) acc = mulmod64(acc, acc >> m.s, m);
         const uint32_t g0 = groups[b], g1 = groups[b + 1];
         for (uint32_t i = g0; i < g1; ++i) acc = mulmod64(acc, chunks[i], m);
     }
-    return mulmod64(acc, one_shifted >> m.s, m);
+    return acc;
+}
+
+// Montgomery form (x * 2^64 mod p) << s of a canonical shifted residue x,
+// given mone = mont_one(m): the checksum terms of the reference.
+PFC_HD uint64_t to_mont(uint64_t x, uint64_t mone, const Modulus& m) {
+    return mulmod64(x, mone >> m.s, m);
 }
 
 // ---------------------------------------------------------------- wheel + small-prime sieve

@@ -591,7 +591,7 @@ static int build_graph(pfc_job* j) {
     for (uint64_t a = j->jbegin; a < j->jend && !rc; a += chunk) {
         const uint32_t b = (uint32_t)std::min<uint64_t>(j->jend, a + chunk);
         rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, d.nmin, j->d_p, j->d_v,
-                              j->d_one, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
+                              j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
                               cap, j->tuning.primes_per_lane, j->tuning.block_size, st);
     }
     if (!rc)
@@ -645,7 +645,7 @@ static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
     for (uint64_t a = j->jbegin; a < j->jend; a += chunk, ++launches) {
         const uint32_t b = (uint32_t)std::min<uint64_t>(j->jend, a + chunk);
         rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, d.nmin, j->d_p, j->d_v,
-                              j->d_one, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
+                              j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
                               cap, j->tuning.primes_per_lane, j->tuning.block_size, st);
         if (rc) return rc;
     }
@@ -859,7 +859,7 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
                 if (rc) break;
                 (void)hipEventRecord(j->ev0, st);
                 rc = pfc::launch_walk(d.modes, j->jbegin, j->jbegin + steps, j->d_list, j->d_comp, d.nmin,
-                                      j->d_p, j->d_v, j->d_one, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac,
+                                      j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac,
                                       (uint32_t)j->fcap, cap, ppl, blk, st);
                 (void)hipEventRecord(j->ev1, st);
                 e = hipEventSynchronize(j->ev1);
@@ -885,7 +885,7 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
             (void)hipEventRecord(j->ev0, st);
             for (uint32_t a = 0; a < total && !rc; a += c)
                 rc = pfc::launch_walk(d.modes, j->jbegin + a, j->jbegin + std::min(total, a + c), j->d_list,
-                                      j->d_comp, d.nmin, j->d_p, j->d_v, j->d_one, j->d_ra, j->d_rb, j->d_ctrl,
+                                      j->d_comp, d.nmin, j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl,
                                       j->d_fac, (uint32_t)j->fcap, cap, win.primes_per_lane, win.block_size, st);
             (void)hipEventRecord(j->ev1, st);
             e = hipEventSynchronize(j->ev1);

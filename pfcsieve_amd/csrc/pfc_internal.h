@@ -50,7 +50,7 @@ int launch_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uin
                  const pfc_setup_table* b, uint32_t cap, hipStream_t st);
 int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list,
                 const uint32_t* d_comp, uint32_t nbase, const uint64_t* d_p, const uint32_t* d_v,
-                const uint64_t* d_one, uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
+                uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
                 pfc_factor* d_fac, uint32_t fcap, uint32_t cap, uint32_t ppl, uint32_t block,
                 hipStream_t st);
 int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint32_t* d_v,

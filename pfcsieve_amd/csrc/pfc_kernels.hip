@@ -11,7 +11,7 @@
 //
 // Device state of a segment is structure-of-arrays, one entry per candidate:
 // p (u64), v (u32 reciprocal), one (u64, R mod p shifted), res_a, res_b (u64
-// shifted residues).  Candidates are appended in no particular order; every
+// plain shifted residues X << s).  Candidates are appended in no particular order; every
 // later result (sums, sorted factor lists) is order independent.
 #include <hip/hip_runtime.h>
 
@@ -149,8 +149,7 @@ __global__ void k_small_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint32_t
 // ------------------------------------------------------------------ setup
 
 // Start residue of one table: Horner over exponent bits, high bit first,
-// acc = acc^2 * prod(chunks of group b), starting from 1; returned multiplied
-// by R mod p so that it is the Montgomery form (X*R mod p) << s.
+// acc = acc^2 * prod(chunks of group b), starting from 1: (X mod p) << s.
 // The tables are uniform across the wavefront and are read with scalar loads.
 typedef const uint32_t __attribute__((address_space(4)))* ConstU32;
 typedef const uint64_t __attribute__((address_space(4)))* ConstU64;
@@ -158,8 +157,8 @@ typedef uint32_t U32x16 __attribute__((ext_vector_type(16)));
 typedef const U32x16 __attribute__((address_space(4)))* ConstU32x16;
 
 __device__ __forceinline__ uint64_t horner_start(const uint64_t* chunks, const uint32_t* groups,
-                                                 uint32_t nbits, const Modulus& m, uint64_t one) {
-    return pfc::start_residue((ConstU64)chunks, (ConstU32)groups, nbits, m, one);
+                                                 uint32_t nbits, const Modulus& m) {
+    return pfc::start_residue((ConstU64)chunks, (ConstU32)groups, nbits, m);
 }
 
 __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
@@ -175,11 +174,10 @@ __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint64_t p = P[i];
         const Modulus m = pfc::make_modulus(p);
-        const uint64_t one = pfc::mont_one(m);
         V[i] = m.v;
-        ONE[i] = one;
-        RA[i] = horner_start(ca, ga, nba, m, one);
-        if (RB) RB[i] = horner_start(cb, gb, nbb, m, one);
+        ONE[i] = pfc::mont_one(m);
+        RA[i] = horner_start(ca, ga, nba, m);
+        if (RB) RB[i] = horner_start(cb, gb, nbb, m);
     }
 }
 
@@ -223,7 +221,7 @@ __global__ __launch_bounds__(64) void k_verify_finish(const uint64_t* __restrict
     for (uint32_t i = 0; i < PFC_VERIFY_BLOCKS; ++i) x = pfc::mulmod64(x, work[2 + i], m);
     const uint64_t one = pfc::mont_one(m);
     work[0] = pfc::mulmod64(one, x, m);                  // X * R mod p
-    work[1] = horner_start(chunks, groups, nbits, m, one);
+    work[1] = pfc::to_mont(horner_start(chunks, groups, nbits, m), one, m);
 }
 
 // ------------------------------------------------------------------ walk
@@ -247,7 +245,6 @@ struct WalkArgs {
     uint32_t nbase;
     const uint64_t* P;
     const uint32_t* V;
-    const uint64_t* ONE;
     uint64_t* RA;
     uint64_t* RB;
     pfc_ctrl* ctrl;
@@ -262,50 +259,45 @@ struct WalkArgs {
 // composite n also multiplies res_b), 3 = C (n = j, composite n only).
 // Composite flags come from a bitmap over n - nbase, one 32-bit word per 32
 // steps.
+//
+// Residues are plain shifted forms x = X << s, so +1 and -1 are one = 2^s and
+// d - one.  A step leaves x == X or x == X + d (the rare second correction of
+// the 3-by-2 division not applied); pfc::walk_attention finds both the +-1
+// residues and the unreduced ones with one add and one compare.
 template <int K>
 struct Lanes {
     uint32_t k1[K], k0[K], v[K], s[K];    // k1, k0: pfc::walk_k1 / walk_k0 of d
-    uint64_t d[K], nd[K], one[K], nmo[K]; // nd = 2^64 - d
+    uint64_t d[K], nd[K], c[K], thr[K];   // nd = 2^64 - d, c = ~one, thr = d - 2*one - 1
     bool live[K];
 };
 
 template <int K>
-__device__ __forceinline__ void lanes_from_state(Lanes<K>& L, uint64_t p, uint32_t v, uint64_t one,
-                                                 int k) {
+__device__ __forceinline__ void lanes_from_state(Lanes<K>& L, uint64_t p, uint32_t v, int k) {
     L.s[k] = pfc::clz64(p);
     L.d[k] = p << L.s[k];
     L.k1[k] = pfc::walk_k1((uint32_t)(L.d[k] >> 32), (uint32_t)L.d[k]);
     L.k0[k] = pfc::walk_k0((uint32_t)L.d[k]);
     L.nd[k] = 0ull - L.d[k];
     L.v[k] = v;
-    L.one[k] = one;
-    L.nmo[k] = L.d[k] - one;
+    const uint64_t one = 1ull << L.s[k];
+    L.c[k] = ~one;
+    L.thr[k] = L.d[k] - 2 * one - 1;
 }
 
-// r[k] <- r[k] * n mod d[k] for all k; returns true, uniformly across the
-// wavefront, if any residue of any lane is +-1.  The two compares are
-// balloted separately so that the no-hit case costs one scalar OR and one
-// branch (no exec-mask save/restore); the record path re-tests each lane.
+// r[k] <- r[k] * n mod d[k] up to the rare correction, for all k; returns
+// true, uniformly across the wavefront, if some residue is +-1 or unreduced
+// (then settle() must run before the next step).  One add and one compare
+// per residue; the no-hit case costs one scalar compare and branch.
 template <int K>
 __device__ __forceinline__ bool step(uint64_t (&r)[K], uint32_t n, const Lanes<K>& L) {
-    bool fix = false;
+    uint64_t rare = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         bool f;
         r[k] = pfc::mulmod32_nofix(r[k], n, L.k1[k], L.k0[k], L.nd[k], L.d[k], L.v[k], &f);
-        fix |= f;
+        rare |= __ballot(pfc::walk_attention(r[k], L.c[k], L.thr[k]));
     }
-    // The second correction of the 3-by-2 division is rare: take it as a
-    // wavefront-uniform branch instead of selects on every step.
-    if (__builtin_expect(__ballot(fix) != 0, 0)) {
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (r[k] >= L.d[k]) r[k] -= L.d[k];
-    }
-    uint64_t hits = 0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) hits |= __ballot(r[k] == L.one[k]) | __ballot(r[k] == L.nmo[k]);
-    return __builtin_expect(hits != 0, 0);
+    return __builtin_expect(rare != 0, 0);
 }
 
 // Per-lane re-test and record, out of line: the walk reaches it only through
@@ -317,13 +309,25 @@ __device__ __noinline__ void record_lane(uint64_t r, uint64_t one, uint64_t nmo,
     if (live && (r == one || r == nmo)) record_factor(fac, fcap, ctrl, d >> s, n, r == nmo, type);
 }
 
+// The slow path of a step: applies the second correction where it is due,
+// then records every residue that is +-1.
 template <int K>
-__device__ __forceinline__ void record_hits(const uint64_t (&r)[K], uint32_t n, int32_t type,
+__device__ __forceinline__ void record_hits(uint64_t (&r)[K], uint32_t n, int32_t type,
                                             const Lanes<K>& L, pfc_factor* fac, uint32_t fcap,
                                             pfc_ctrl* ctrl) {
+    uint64_t hits = 0;
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-        record_lane(r[k], L.one[k], L.nmo[k], L.d[k], L.s[k], L.live[k], n, type, fac, fcap, ctrl);
+    for (int k = 0; k < K; ++k) {
+        if (r[k] >= L.d[k]) r[k] -= L.d[k];
+        const uint64_t one = ~L.c[k];
+        hits |= __ballot(r[k] == one) | __ballot(r[k] == L.d[k] - one);
+    }
+    if (hits) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            record_lane(r[k], ~L.c[k], L.d[k] + L.c[k] + 1, L.d[k], L.s[k], L.live[k], n, type, fac,
+                        fcap, ctrl);
+    }
 }
 
 struct WalkStream {
@@ -457,7 +461,7 @@ __global__ __launch_bounds__(256) void k_walk(const WalkArgs A) {
         const uint32_t idx = first + k;
         L.live[k] = idx < n_c;
         const uint32_t src = L.live[k] ? idx : first;
-        lanes_from_state<K>(L, A.P[src], A.V[src], A.ONE[src], k);
+        lanes_from_state<K>(L, A.P[src], A.V[src], k);
         ra[k] = A.RA[src];
         rb[k] = (MODE == 2) ? A.RB[src] : 0;
     }
@@ -474,13 +478,15 @@ __global__ __launch_bounds__(256) void k_walk(const WalkArgs A) {
 // ------------------------------------------------------------------ check
 
 // Term per candidate (check.cl:59,115,161): F or C: Mont(X) + Mont(nmax-1);
-// F|C: Mont(F) + Mont(C) + Mont(nmax-1); P: Mont(X).
+// F|C: Mont(F) + Mont(C) + Mont(nmax-1); P: Mont(X).  ra, rb are the walk's
+// plain shifted residues; mone = pfc::mont_one(m).
 __device__ __forceinline__ uint64_t check_term(uint32_t modes, uint32_t nmax, const Modulus& m,
-                                               uint64_t one, uint64_t ra, uint64_t rb) {
-    const uint64_t a = ra >> m.s;
+                                               uint64_t mone, uint64_t ra, uint64_t rb) {
+    const uint64_t a = pfc::to_mont(ra, mone, m) >> m.s;
     if (modes & PFC_MODE_PRIMORIAL) return a;
-    uint64_t t = a + (pfc::mulmod32(one, nmax - 1, m) >> m.s);
-    if ((modes & PFC_MODE_FACTORIAL) && (modes & PFC_MODE_COMPOSITORIAL)) t += rb >> m.s;
+    uint64_t t = a + (pfc::mulmod32(mone, nmax - 1, m) >> m.s);
+    if ((modes & PFC_MODE_FACTORIAL) && (modes & PFC_MODE_COMPOSITORIAL))
+        t += pfc::to_mont(rb, mone, m) >> m.s;
     return t;
 }
 
@@ -537,7 +543,7 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
     if (n_c > J.cap) n_c = J.cap;
     const uint32_t first = it.first + __lane_id() * K;
     Lanes<K> L;
-    uint64_t ra[K], rb[K];
+    uint64_t ra[K], rb[K], mone[K];
     const bool two = J.modes == (PFC_MODE_FACTORIAL | PFC_MODE_COMPOSITORIAL);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -547,12 +553,13 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
         const uint64_t p = J.P[src];
         if (it.flags & kBatchSetup) {
             const Modulus m = pfc::make_modulus(p);
-            const uint64_t one = pfc::mont_one(m);
-            lanes_from_state<K>(L, p, m.v, one, k);
-            ra[k] = horner_start(J.ca, J.ga, J.nba, m, one);
-            rb[k] = two ? horner_start(J.cb, J.gb, J.nbb, m, one) : 0;
+            mone[k] = pfc::mont_one(m);
+            lanes_from_state<K>(L, p, m.v, k);
+            ra[k] = horner_start(J.ca, J.ga, J.nba, m);
+            rb[k] = two ? horner_start(J.cb, J.gb, J.nbb, m) : 0;
         } else {
-            lanes_from_state<K>(L, p, J.V[src], J.ONE[src], k);
+            mone[k] = 0;
+            lanes_from_state<K>(L, p, J.V[src], k);
             ra[k] = J.RA[src];
             rb[k] = two ? J.RB[src] : 0;
         }
@@ -573,7 +580,7 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
         if (two) J.RB[idx] = rb[k];
         if (it.flags & kBatchSetup) {
             J.V[idx] = L.v[k];
-            J.ONE[idx] = L.one[k];
+            J.ONE[idx] = mone[k];
         }
         if (it.flags & kBatchCheck) {
             Modulus m;
@@ -581,7 +588,8 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
             m.d0 = (uint32_t)L.d[k];
             m.v = L.v[k];
             m.s = L.s[k];
-            sum += check_term(J.modes, J.nmax, m, L.one[k], ra[k], rb[k]);
+            sum += check_term(J.modes, J.nmax, m, (it.flags & kBatchSetup) ? mone[k] : J.ONE[idx],
+                              ra[k], rb[k]);
         }
     }
     if (it.flags & kBatchCheck) {
@@ -738,13 +746,13 @@ static void walk_launch(const WalkArgs& A, uint32_t block, hipStream_t st) {
 
 int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list,
                 const uint32_t* d_comp, uint32_t nbase, const uint64_t* d_p, const uint32_t* d_v,
-                const uint64_t* d_one, uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
+                uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
                 pfc_factor* d_fac, uint32_t fcap, uint32_t cap, uint32_t ppl, uint32_t block,
                 hipStream_t st) {
     if (j1 <= j0 || cap == 0) return PFC_OK;
     WalkArgs A;
     A.j0 = j0; A.j1 = j1; A.list = d_list; A.composite = d_comp; A.nbase = nbase;
-    A.P = d_p; A.V = d_v; A.ONE = d_one; A.RA = d_ra; A.RB = d_rb; A.ctrl = d_ctrl;
+    A.P = d_p; A.V = d_v; A.RA = d_ra; A.RB = d_rb; A.ctrl = d_ctrl;
     A.fac = d_fac; A.fcap = fcap; A.cap = cap;
     if (block != 64 && block != 128 && block != 256) block = 256;
     int mode;
@@ -888,13 +896,12 @@ int pfc_stage_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, 
 
 int pfc_stage_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list,
                    const uint32_t* d_composite, uint32_t nbase, const uint64_t* d_p,
-                   const uint32_t* d_v, const uint64_t* d_one, uint64_t* d_res_a,
-                   uint64_t* d_res_b, pfc_ctrl* d_ctrl, pfc_factor* d_fac, uint32_t fcap,
-                   uint32_t cap, void* stream) {
+                   const uint32_t* d_v, uint64_t* d_res_a, uint64_t* d_res_b, pfc_ctrl* d_ctrl,
+                   pfc_factor* d_fac, uint32_t fcap, uint32_t cap, void* stream) {
     pfc::g_err[0] = 0;
-    if (!d_p || !d_v || !d_one || !d_res_a || !d_ctrl || (!d_fac && fcap))
+    if (!d_p || !d_v || !d_res_a || !d_ctrl || (!d_fac && fcap))
         return pfc::set_error(PFC_ERR_INVALID, "walk: null pointer");
-    return pfc::launch_walk(modes, j0, j1, d_list, d_composite, nbase, d_p, d_v, d_one, d_res_a,
+    return pfc::launch_walk(modes, j0, j1, d_list, d_composite, nbase, d_p, d_v, d_res_a,
                             d_res_b, d_ctrl, d_fac, fcap, cap, 4, 256, (hipStream_t)stream);
 }
 

@@ -21,8 +21,29 @@ int t_modulus_ok(uint64_t p) { return pfc::modulus_ok(p, pfc::make_modulus(p)); 
 int t_strong_prp2(uint64_t n) { return pfc::strong_prp2(n); }
 int t_is_prime_u32(uint32_t n) { return pfc::is_prime_u32(n); }
 uint64_t t_start_residue(const uint64_t* chunks, const uint32_t* groups, uint32_t nbits, uint64_t p) {
+    return pfc::start_residue(chunks, groups, nbits, pfc::make_modulus(p));
+}
+uint64_t t_to_mont(uint64_t p, uint64_t x_shifted) {
     const pfc::Modulus m = pfc::make_modulus(p);
-    return pfc::start_residue(chunks, groups, nbits, m, pfc::mont_one(m));
+    return pfc::to_mont(x_shifted, pfc::mont_one(m), m);
+}
+// One walk step as the kernels take it: the 3-by-2 step without its second
+// correction (*raw), the attention test on it (*attention), then the
+// correction; returns the canonical residue.
+uint64_t t_walk_step(uint64_t p, uint64_t a_shifted, uint32_t n, uint64_t* raw, int* attention) {
+    const pfc::Modulus m = pfc::make_modulus(p);
+    const uint64_t d = pfc::mod_d(m), one = 1ull << m.s;
+    bool fix;
+    uint64_t x = pfc::mulmod32_nofix(a_shifted, n, pfc::walk_k1(m.d1, m.d0), pfc::walk_k0(m.d0), 0ull - d,
+                                     d, m.v, &fix);
+    *raw = x;
+    *attention = pfc::walk_attention(x, ~one, d - 2 * one - 1);
+    return x >= d ? x - d : x;
+}
+int t_walk_attention(uint64_t p, uint64_t x) {
+    const pfc::Modulus m = pfc::make_modulus(p);
+    const uint64_t d = pfc::mod_d(m), one = 1ull << m.s;
+    return pfc::walk_attention(x, ~one, d - 2 * one - 1);
 }
 // Candidates of [lo, hi) (lo >= 114) exactly as k_generate's lanes compute them.
 size_t t_generate(uint64_t lo, uint64_t hi, uint64_t* out, size_t cap) {

@@ -162,6 +162,11 @@ def A():
     L.t_generate.restype = C.c_size_t
     L.t_start_residue.argtypes = [C.POINTER(u64), C.POINTER(u32), u32, u64]
     L.t_start_residue.restype = u64
+    L.t_to_mont.argtypes = [u64, u64]
+    L.t_to_mont.restype = u64
+    L.t_walk_step.argtypes = [u64, u64, u32, C.POINTER(u64), C.POINTER(C.c_int)]
+    L.t_walk_step.restype = u64
+    L.t_walk_attention.argtypes = [u64, u64]
     return L
 
 
@@ -195,6 +200,56 @@ def test_mulmod_against_bigint(A):
             b = rng.randrange(R)
             assert A.t_mulmod64(p, a << s, b) == (a * b % p) << s
             assert A.t_sqrmod(p, a << s) == (a * a % p) << s
+            assert A.t_to_mont(p, a << s) == (a * R % p) << s
+
+
+def test_walk_attention_bounds(A):
+    """The walk's one-compare test passes exactly the x with x <= 2^s or
+    x >= d - 2^s, checked at both edges of both intervals and at random x."""
+    rng = random.Random(11)
+    for p in _primes_mix(rng, 500):
+        s = 64 - p.bit_length()
+        d, one = p << s, 1 << s
+        edges = [0, 1, one - 1, one, one + 1, d - one - 1, d - one, d - one + 1, d - 1, d, d + 1, R - 1]
+        for x in edges + [rng.randrange(R) for _ in range(20)] + [rng.randrange(d) for _ in range(20)]:
+            x %= R
+            assert A.t_walk_attention(p, x) == int(x <= one or x >= d - one), (p, x)
+
+
+def test_walk_step_against_bigint(A):
+    """One walk step (3-by-2 step, attention test, correction) against big
+    integers: the settled residue is a*n mod p; the raw one is that or that
+    plus d; every raw value that is +-1 or unreduced raises attention; and
+    steps landing exactly on +1 and -1 (n chosen as a quotient of residues)
+    are caught."""
+    rng = random.Random(12)
+    raw, att = C.c_uint64(), C.c_int()
+    # (p, a, n) whose raw result needs the second correction (found by a
+    # random search over odd moduli and 32-bit n; it is far rarer for n below
+    # 2^24 and random moduli).
+    for p, a, n in [(11808775468092489729, 10992610456871730286, 4294967044),
+                    (9248424854364649175, 5733249779935437510, 4294967082),
+                    (9264490322874859521, 4812142961238367030, 3106929992),
+                    (9275621306045530121, 5000254257389108843, 3738354044),
+                    (9583256094208529071, 5956682225011036978, 3496521807)]:
+        got = A.t_walk_step(p, a, n, C.byref(raw), C.byref(att))
+        assert raw.value >= p and att.value and got == a * n % p, (p, a, n)
+    for p in _primes_mix(rng, 1500):
+        s = 64 - p.bit_length()
+        d = p << s
+        cases = [(rng.randrange(1, p), rng.randrange(1, min(p, 1 << 32))) for _ in range(30)]
+        for target in (1, p - 1):
+            for _ in range(4):
+                n = rng.randrange(1, min(p, 1 << 32))
+                if math.gcd(n, p) == 1:    # the moduli of the mix are odd, not all prime
+                    cases.append((target * pow(n, -1, p) % p, n))
+        for a, n in cases:
+            got = A.t_walk_step(p, a << s, n, C.byref(raw), C.byref(att))
+            want = a * n % p
+            assert got == want << s, (p, a, n)
+            assert raw.value in (want << s, (want << s) + d), (p, a, n)
+            if raw.value >= d or want in (1, p - 1):
+                assert att.value, (p, a, n)
 
 
 def test_strong_prp2_device_arith(A, orc):
@@ -244,8 +299,8 @@ def test_generate_window_logic_matches_oracle(A, orc, lo, hi):
 
 @pytest.mark.parametrize("kind,N", [(0, 100), (0, 4999), (0, 99999), (1, 100), (1, 77777), (2, 100), (2, 65432)])
 def test_start_residue_against_bigint(L, A, kind, N):
-    """Horner's rule over the table in the shifted representation, then the
-    product by R, gives (X * 2^64 mod p) << s for X = N!, N#, N!/N#."""
+    """Horner's rule over the table in the shifted representation gives
+    (X mod p) << s for X = N!, N#, N!/N#."""
     nch, nb = C.c_size_t(), C.c_uint32()
     L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb))
     ch = (C.c_uint64 * max(nch.value, 1))()
@@ -257,4 +312,4 @@ def test_start_residue_against_bigint(L, A, kind, N):
     for p in _primes_mix(rng, 40) + [101, 103, 10007, N + 1 if (N + 1) % 2 else N + 2]:
         s = 64 - p.bit_length()
         got = A.t_start_residue(ch, gr, nb.value, p)
-        assert got == (X % p) * R % p << s, (kind, N, p)
+        assert got == (X % p) << s, (kind, N, p)
