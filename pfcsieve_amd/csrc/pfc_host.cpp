@@ -572,6 +572,15 @@ static int drain_segment(pfc_job* j, bool* overflow, uint64_t launches, bool tim
 
 // Captures clear -> primes below 114 -> generate -> setup -> walk chunks ->
 // check for a job as one HIP graph; segment bounds come from device memory.
+// Largest multiplier of a job's walk steps below b: n = j, or in P mode the
+// j-th prime of the ascending list.  The walk pairs its steps (tests every
+// second one) on wavefronts whose p all exceed twice this bound.
+static uint32_t walk_bound(const pfc_job* j, uint64_t b) {
+    if (b == 0) return 0;
+    if (j->desc.modes == PFC_MODE_PRIMORIAL) return b <= j->list.size() ? j->list[b - 1] : 0;
+    return (uint32_t)(b - 1);
+}
+
 static int build_graph(pfc_job* j) {
     hipStream_t st = j->stream;
     const uint32_t cap = j->plan.cap;
@@ -592,7 +601,7 @@ static int build_graph(pfc_job* j) {
         const uint32_t b = (uint32_t)std::min<uint64_t>(j->jend, a + chunk);
         rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, d.nmin, j->d_p, j->d_v,
                               j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
-                              cap, j->tuning.primes_per_lane, j->tuning.block_size, st);
+                              cap, j->tuning.primes_per_lane, j->tuning.block_size, walk_bound(j, b), st);
     }
     if (!rc)
         rc = pfc::launch_check(d.modes, d.nmax, j->d_p, j->d_v, j->d_one, j->d_ra, j->d_rb, j->d_ctrl,
@@ -646,7 +655,7 @@ static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
         const uint32_t b = (uint32_t)std::min<uint64_t>(j->jend, a + chunk);
         rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, d.nmin, j->d_p, j->d_v,
                               j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
-                              cap, j->tuning.primes_per_lane, j->tuning.block_size, st);
+                              cap, j->tuning.primes_per_lane, j->tuning.block_size, walk_bound(j, b), st);
         if (rc) return rc;
     }
     (void)hipEventRecord(j->evw1, st);
@@ -860,7 +869,7 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
                 (void)hipEventRecord(j->ev0, st);
                 rc = pfc::launch_walk(d.modes, j->jbegin, j->jbegin + steps, j->d_list, j->d_comp, d.nmin,
                                       j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac,
-                                      (uint32_t)j->fcap, cap, ppl, blk, st);
+                                      (uint32_t)j->fcap, cap, ppl, blk, walk_bound(j, j->jbegin + steps), st);
                 (void)hipEventRecord(j->ev1, st);
                 e = hipEventSynchronize(j->ev1);
                 if (e != hipSuccess) rc = hip_fail(e, "autotune walk");
@@ -886,7 +895,8 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
             for (uint32_t a = 0; a < total && !rc; a += c)
                 rc = pfc::launch_walk(d.modes, j->jbegin + a, j->jbegin + std::min(total, a + c), j->d_list,
                                       j->d_comp, d.nmin, j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl,
-                                      j->d_fac, (uint32_t)j->fcap, cap, win.primes_per_lane, win.block_size, st);
+                                      j->d_fac, (uint32_t)j->fcap, cap, win.primes_per_lane, win.block_size,
+                                      walk_bound(j, j->jbegin + std::min(total, a + c)), st);
             (void)hipEventRecord(j->ev1, st);
             e = hipEventSynchronize(j->ev1);
             if (e != hipSuccess && !rc) rc = hip_fail(e, "autotune walk chunk");
@@ -1147,7 +1157,9 @@ extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** d
         }
         const double per = (double)(j1 - j0) * ((j->d_rb) ? 1.9 : 1.0) +
                            ((flags & 1u) ? 2.2 * (double)(j->ta.chunks.size() + j->tb.chunks.size()) : 0.0);
-        for (uint32_t f = 0; f < s.ncand; f += 64 * K) items.push_back({{(uint32_t)i, f, j0, j1, flags, {0, 0, 0}}, per});
+        const uint32_t bmax = walk_bound(j, j1);
+        for (uint32_t f = 0; f < s.ncand; f += 64 * K)
+            items.push_back({{(uint32_t)i, f, j0, j1, flags, bmax, {0, 0}}, per});
         s.jcur = j1;
         if (j1 == j->jend) finishing[i] = 1;
     }

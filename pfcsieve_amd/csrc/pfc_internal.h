@@ -29,9 +29,10 @@ struct pfc_batch_job {
 };
 
 // One work item: 64*K consecutive candidates of a job from index `first`,
-// multiplier steps [j0, j1); flags bit0 = set up first, bit1 = fold checksum.
+// multiplier steps [j0, j1) whose largest multiplier is bmax (0: not known);
+// flags bit0 = set up first, bit1 = fold checksum.
 struct pfc_batch_item {
-    uint32_t job, first, j0, j1, flags, pad[3];
+    uint32_t job, first, j0, j1, flags, bmax, pad[2];
 };
 
 namespace pfc {
@@ -52,7 +53,7 @@ int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list
                 const uint32_t* d_comp, uint32_t nbase, const uint64_t* d_p, const uint32_t* d_v,
                 uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
                 pfc_factor* d_fac, uint32_t fcap, uint32_t cap, uint32_t ppl, uint32_t block,
-                hipStream_t st);
+                uint32_t bmax, hipStream_t st);
 int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint32_t* d_v,
                  const uint64_t* d_one, const uint64_t* d_ra, const uint64_t* d_rb,
                  pfc_ctrl* d_ctrl, uint32_t cap, hipStream_t st);

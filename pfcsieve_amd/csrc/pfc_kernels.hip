@@ -240,6 +240,7 @@ __device__ __forceinline__ void record_factor(pfc_factor* fac, uint32_t fcap, pf
 
 struct WalkArgs {
     uint32_t j0, j1;                  // multiplier steps [j0, j1)
+    uint32_t bmax;                    // largest multiplier of those steps (0: not known)
     const uint32_t* list;             // P, C: multipliers
     const uint32_t* composite;        // F|C: composite bitmap, bit (n - nbase)
     uint32_t nbase;
@@ -264,11 +265,22 @@ struct WalkArgs {
 // d - one.  A step leaves x == X or x == X + d (the rare second correction of
 // the 3-by-2 division not applied); pfc::walk_attention finds both the +-1
 // residues and the unreduced ones with one add and one compare.
+//
+// Paired steps.  When every multiplier of a launch is at most bmax and every
+// p of a wavefront exceeds 2 * bmax, only every second step of a chain is
+// tested.  A residue that is +-1 after step m is +-n after the next step n,
+// and 1 <= n <= bmax < p/2, so the second step's test, widened to the band
+// x <= B or x >= d - B with B = bmax << s (the same add and compare with
+// other constants), also finds the hits of the untested step; the slow path
+// then compares x exactly with +-1 (a hit at n) and +-n << s (a hit at m).
+// The untested result may stay unreduced: the 3-by-2 step accepts any 64-bit
+// input because n < 2^31.  The band passes a fraction 2 * bmax / p of the
+// residues.  Wavefronts with a smaller p test every step against +-1.
 template <int K>
 struct Lanes {
     uint32_t k1[K], k0[K], v[K], s[K];    // k1, k0: pfc::walk_k1 / walk_k0 of d
-    uint64_t d[K], nd[K], c[K], thr[K];   // nd = 2^64 - d, c = ~one, thr = d - 2*one - 1
-    bool live[K];
+    uint64_t d[K], nd[K], c[K], thr[K];   // nd = 2^64 - d; test: c = ~b, thr = d - 2*b - 1
+    bool live[K];                         //   with b = one, or b = B for paired steps
 };
 
 template <int K>
@@ -284,49 +296,97 @@ __device__ __forceinline__ void lanes_from_state(Lanes<K>& L, uint64_t p, uint32
     L.thr[k] = L.d[k] - 2 * one - 1;
 }
 
-// r[k] <- r[k] * n mod d[k] up to the rare correction, for all k; returns
-// true, uniformly across the wavefront, if some residue is +-1 or unreduced
-// (then settle() must run before the next step).  One add and one compare
-// per residue; the no-hit case costs one scalar compare and branch.
+// True, uniformly, when the wavefront may take paired steps for multipliers
+// up to bmax (0 = unknown bound: never); then switches its test to the band.
 template <int K>
-__device__ __forceinline__ bool step(uint64_t (&r)[K], uint32_t n, const Lanes<K>& L) {
-    uint64_t rare = 0;
+__device__ __forceinline__ bool pair_steps(Lanes<K>& L, uint32_t bmax) {
+    if (bmax == 0) return false;
+    bool small = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) small |= (L.d[k] >> L.s[k]) <= 2ull * bmax;
+    if (__ballot(small) != 0) return false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t B = (uint64_t)bmax << L.s[k];
+        L.c[k] = ~B;
+        L.thr[k] = L.d[k] - 2 * B - 1;
+    }
+    return true;
+}
+
+// r[k] <- r[k] * n mod d[k] up to the rare correction, for all k.
+template <int K>
+__device__ __forceinline__ void mul_step(uint64_t (&r)[K], uint32_t n, const Lanes<K>& L) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         bool f;
         r[k] = pfc::mulmod32_nofix(r[k], n, L.k1[k], L.k0[k], L.nd[k], L.d[k], L.v[k], &f);
-        rare |= __ballot(pfc::walk_attention(r[k], L.c[k], L.thr[k]));
     }
+}
+
+// An untested step's result: the empty asm makes the compiler treat it as a
+// fresh value, which keeps it out of the temporaries of the next product
+// (without it the allocator copies half of it aside on every pair).
+template <int K>
+__device__ __forceinline__ void untested(uint64_t (&r)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) asm volatile("" : "+v"(r[k]));
+}
+
+// True, uniformly across the wavefront, if some residue passes the lanes'
+// test (then settle() must run before the next step).  One add and one
+// compare per residue; the no-hit case costs one scalar compare and branch.
+template <int K>
+__device__ __forceinline__ bool attention(const uint64_t (&r)[K], const Lanes<K>& L) {
+    uint64_t rare = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) rare |= __ballot(pfc::walk_attention(r[k], L.c[k], L.thr[k]));
     return __builtin_expect(rare != 0, 0);
+}
+
+template <int K>
+__device__ __forceinline__ bool step(uint64_t (&r)[K], uint32_t n, const Lanes<K>& L) {
+    mul_step<K>(r, n, L);
+    return attention<K>(r, L);
 }
 
 // Per-lane re-test and record, out of line: the walk reaches it only through
 // the wavefront-uniform hit branch, so the compiler cannot fold the per-lane
-// condition back into an exec-masked region on the hot path.
-__device__ __noinline__ void record_lane(uint64_t r, uint64_t one, uint64_t nmo, uint64_t d, uint32_t s,
-                                         bool live, uint32_t n, int32_t type, pfc_factor* fac,
-                                         uint32_t fcap, pfc_ctrl* ctrl) {
-    if (live && (r == one || r == nmo)) record_factor(fac, fcap, ctrl, d >> s, n, r == nmo, type);
+// condition back into an exec-masked region on the hot path.  r is canonical;
+// +-1 is a hit at step n, and (m != 0) +-n << s a hit at the untested step m.
+__device__ __noinline__ void record_lane(uint64_t r, uint64_t d, uint32_t s, bool live, uint32_t m,
+                                         uint32_t n, int32_t type, pfc_factor* fac, uint32_t fcap,
+                                         pfc_ctrl* ctrl) {
+    if (!live) return;
+    const uint64_t one = 1ull << s;
+    if (r == one || r == d - one) record_factor(fac, fcap, ctrl, d >> s, n, r != one, type);
+    if (m) {
+        const uint64_t b = (uint64_t)n << s;
+        if (r == b || r == d - b) record_factor(fac, fcap, ctrl, d >> s, m, r != b, type);
+    }
 }
 
-// The slow path of a step: applies the second correction where it is due,
-// then records every residue that is +-1.
+// The slow path of a tested step n (m: the untested step before it, or 0):
+// applies the second correction where it is due, then records the hits.
 template <int K>
-__device__ __forceinline__ void record_hits(uint64_t (&r)[K], uint32_t n, int32_t type,
-                                            const Lanes<K>& L, pfc_factor* fac, uint32_t fcap,
-                                            pfc_ctrl* ctrl) {
+__device__ __forceinline__ void settle(uint64_t (&r)[K], uint32_t m, uint32_t n, int32_t type,
+                                       const Lanes<K>& L, pfc_factor* fac, uint32_t fcap,
+                                       pfc_ctrl* ctrl) {
     uint64_t hits = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (r[k] >= L.d[k]) r[k] -= L.d[k];
-        const uint64_t one = ~L.c[k];
+        const uint64_t one = 1ull << L.s[k];
         hits |= __ballot(r[k] == one) | __ballot(r[k] == L.d[k] - one);
+        if (m) {
+            const uint64_t b = (uint64_t)n << L.s[k];
+            hits |= __ballot(r[k] == b) | __ballot(r[k] == L.d[k] - b);
+        }
     }
     if (hits) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            record_lane(r[k], ~L.c[k], L.d[k] + L.c[k] + 1, L.d[k], L.s[k], L.live[k], n, type, fac,
-                        fcap, ctrl);
+            record_lane(r[k], L.d[k], L.s[k], L.live[k], m, n, type, fac, fcap, ctrl);
     }
 }
 
@@ -339,23 +399,56 @@ struct WalkStream {
     pfc_ctrl* ctrl;
 };
 
-template <int MODE, int K>
+// One tested step of a chain, alone (no untested step before it).
+template <int K>
+__device__ __forceinline__ void single(uint64_t (&r)[K], uint32_t n, int32_t type, const Lanes<K>& L,
+                                       const WalkStream& W) {
+    if (step<K>(r, n, L)) settle<K>(r, 0, n, type, L, W.fac, W.fcap, W.ctrl);
+}
+
+// Steps m (untested) and n (tested) of a chain.
+template <int K>
+__device__ __forceinline__ void pair(uint64_t (&r)[K], uint32_t m, uint32_t n, int32_t type,
+                                     const Lanes<K>& L, const WalkStream& W) {
+    mul_step<K>(r, m, L);
+    untested<K>(r);
+    if (step<K>(r, n, L)) settle<K>(r, m, n, type, L, W.fac, W.fcap, W.ctrl);
+}
+
+// A step of a chain whose steps come one at a time (composite multipliers):
+// with pairs, *pend holds the untested step before n (0 = none), uniformly.
+template <int K, bool PAIRED>
+__device__ __forceinline__ void chain_step(uint64_t (&r)[K], uint32_t n, uint32_t* pend, int32_t type,
+                                           const Lanes<K>& L, const WalkStream& W) {
+    mul_step<K>(r, n, L);
+    if (PAIRED) untested<K>(r);
+    if (!PAIRED || *pend) {
+        if (attention<K>(r, L)) settle<K>(r, PAIRED ? *pend : 0, n, type, L, W.fac, W.fcap, W.ctrl);
+        *pend = 0;
+    } else {
+        *pend = n;
+    }
+}
+
+template <int MODE, int K, bool PAIRED>
 __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K], const Lanes<K>& L,
                                            uint32_t j0, uint32_t j1, const WalkStream& W) {
     if (MODE == 0) {
         // Four steps per trip: one scalar compare-and-branch per four updates.
         uint32_t j = j0;
         for (; j + 4 <= j1; j += 4) {
+            uint32_t n[4];
 #pragma unroll
-            for (uint32_t t = 0; t < 4; ++t) {
-                const uint32_t n = __builtin_amdgcn_readfirstlane(j + t);
-                if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_FACTORIAL, L, W.fac, W.fcap, W.ctrl);
+            for (uint32_t t = 0; t < 4; ++t) n[t] = __builtin_amdgcn_readfirstlane(j + t);
+            if (PAIRED) {
+                pair<K>(ra, n[0], n[1], PFC_FACTORIAL, L, W);
+                pair<K>(ra, n[2], n[3], PFC_FACTORIAL, L, W);
+            } else {
+#pragma unroll
+                for (uint32_t t = 0; t < 4; ++t) single<K>(ra, n[t], PFC_FACTORIAL, L, W);
             }
         }
-        for (; j < j1; ++j) {
-            const uint32_t n = __builtin_amdgcn_readfirstlane(j);
-            if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_FACTORIAL, L, W.fac, W.fcap, W.ctrl);
-        }
+        for (; j < j1; ++j) single<K>(ra, __builtin_amdgcn_readfirstlane(j), PFC_FACTORIAL, L, W);
     } else if (MODE == 1) {
         // The primes are fetched 16 at a time with one 64-byte scalar block
         // load (s_load_dwordx16) when the list is 64-byte aligned, which
@@ -365,42 +458,48 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
         uint32_t j = j0;
         if ((reinterpret_cast<uintptr_t>(W.list) & 63u) == 0) {
             const uint32_t jal = min(j1, (j0 + 15u) & ~15u);
-            for (; j < jal; ++j) {
-                const uint32_t n = __builtin_amdgcn_readfirstlane(list[j]);
-                if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_PRIMORIAL, L, W.fac, W.fcap, W.ctrl);
-            }
+            for (; j < jal; ++j) single<K>(ra, __builtin_amdgcn_readfirstlane(list[j]), PFC_PRIMORIAL, L, W);
             const ConstU32x16 blocks = (ConstU32x16)W.list;
             for (; j + 16 <= j1; j += 16) {
                 const U32x16 q = blocks[j >> 4];
+                if (PAIRED) {
 #pragma unroll
-                for (int t = 0; t < 16; ++t) {
-                    const uint32_t n = __builtin_amdgcn_readfirstlane(q[t]);
-                    if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_PRIMORIAL, L, W.fac, W.fcap, W.ctrl);
+                    for (int t = 0; t < 16; t += 2)
+                        pair<K>(ra, __builtin_amdgcn_readfirstlane(q[t]), __builtin_amdgcn_readfirstlane(q[t + 1]),
+                                PFC_PRIMORIAL, L, W);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < 16; ++t)
+                        single<K>(ra, __builtin_amdgcn_readfirstlane(q[t]), PFC_PRIMORIAL, L, W);
                 }
             }
         } else {
             for (; j + 16 <= j1; j += 16) {
                 uint32_t q[16];
 #pragma unroll
-                for (int t = 0; t < 16; ++t) q[t] = list[j + t];
+                for (int t = 0; t < 16; ++t) q[t] = __builtin_amdgcn_readfirstlane(list[j + t]);
+                if (PAIRED) {
 #pragma unroll
-                for (int t = 0; t < 16; ++t) {
-                    const uint32_t n = __builtin_amdgcn_readfirstlane(q[t]);
-                    if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_PRIMORIAL, L, W.fac, W.fcap, W.ctrl);
+                    for (int t = 0; t < 16; t += 2) pair<K>(ra, q[t], q[t + 1], PFC_PRIMORIAL, L, W);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < 16; ++t) single<K>(ra, q[t], PFC_PRIMORIAL, L, W);
                 }
             }
         }
-        for (; j < j1; ++j) {
-            const uint32_t n = __builtin_amdgcn_readfirstlane(list[j]);
-            if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_PRIMORIAL, L, W.fac, W.fcap, W.ctrl);
-        }
+        for (; j < j1; ++j) single<K>(ra, __builtin_amdgcn_readfirstlane(list[j]), PFC_PRIMORIAL, L, W);
     } else {
         // F|C (MODE 2) and C (MODE 3): composite flags come from the bitmap,
         // one 32-bit word per 32 steps.  A whole word inside [j0, j1) takes an
         // unrolled path whose flag tests use constant bit indices (one scalar
         // bit-compare and branch per n, no shifts or loop counter); partial
-        // words at either end of the range take the generic loop.
+        // words at either end of the range take the generic loop.  The
+        // composite chain (res_b in F|C, res_a in C) is paired across words
+        // through `pend`; the factorial chain of F|C pairs n, n + 1 inside a
+        // word.
         const ConstU32 comp = (ConstU32)W.composite;
+        uint64_t (&rc)[K] = (MODE == 2) ? rb : ra;
+        uint32_t pend = 0;
         uint32_t j = j0;
         while (j < j1) {
             const uint32_t off = j - W.nbase;
@@ -412,12 +511,15 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
                 for (uint32_t t = 0; t < 32; ++t) {
                     const uint32_t n = __builtin_amdgcn_readfirstlane(nword + t);
                     if (MODE == 2) {
-                        if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_FACTORIAL, L, W.fac, W.fcap, W.ctrl);
-                        if (((word >> t) & 1u) && step<K>(rb, n, L))
-                            record_hits<K>(rb, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
-                    } else if (((word >> t) & 1u) && step<K>(ra, n, L)) {
-                        record_hits<K>(ra, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
+                        if (!PAIRED) single<K>(ra, n, PFC_FACTORIAL, L, W);
+                        else if (t & 1u) {
+                            if (step<K>(ra, n, L)) settle<K>(ra, n - 1, n, PFC_FACTORIAL, L, W.fac, W.fcap, W.ctrl);
+                        } else {
+                            mul_step<K>(ra, n, L);
+                            untested<K>(ra);
+                        }
                     }
+                    if ((word >> t) & 1u) chain_step<K, PAIRED>(rc, n, &pend, PFC_COMPOSITORIAL, L, W);
                 }
                 j = nword + 32u;
                 continue;
@@ -425,18 +527,22 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
             const uint32_t jend = min(j1, nword + 32u);
             for (; j < jend; ++j) {
                 const uint32_t n = __builtin_amdgcn_readfirstlane(j);
-                const bool composite = (word >> ((n - W.nbase) & 31u)) & 1u;
-                if (MODE == 2) {
-                    if (step<K>(ra, n, L)) record_hits<K>(ra, n, PFC_FACTORIAL, L, W.fac, W.fcap, W.ctrl);
-                    if (composite && step<K>(rb, n, L))
-                        record_hits<K>(rb, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
-                } else if (composite) {
-                    if (step<K>(ra, n, L))
-                        record_hits<K>(ra, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
-                }
+                if (MODE == 2) single<K>(ra, n, PFC_FACTORIAL, L, W);
+                if ((word >> ((n - W.nbase) & 31u)) & 1u)
+                    chain_step<K, PAIRED>(rc, n, &pend, PFC_COMPOSITORIAL, L, W);
             }
         }
+        // A last untested composite step is tested alone.
+        if (PAIRED && pend) settle<K>(rc, 0, pend, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
     }
+}
+
+// Walks steps [j0, j1) with pairs where the wavefront allows them.
+template <int MODE, int K>
+__device__ __forceinline__ void walk_range(uint64_t (&ra)[K], uint64_t (&rb)[K], Lanes<K>& L,
+                                           uint32_t j0, uint32_t j1, uint32_t bmax, const WalkStream& W) {
+    if (pair_steps<K>(L, bmax)) walk_steps<MODE, K, true>(ra, rb, L, j0, j1, W);
+    else walk_steps<MODE, K, false>(ra, rb, L, j0, j1, W);
 }
 
 __host__ __device__ constexpr int walk_mode(uint32_t modes) {
@@ -466,7 +572,7 @@ __global__ __launch_bounds__(256) void k_walk(const WalkArgs A) {
         rb[k] = (MODE == 2) ? A.RB[src] : 0;
     }
     const WalkStream W{A.list, A.composite, A.nbase, A.fac, A.fcap, A.ctrl};
-    walk_steps<MODE, K>(ra, rb, L, A.j0, A.j1, W);
+    walk_range<MODE, K>(ra, rb, L, A.j0, A.j1, A.bmax, W);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (!L.live[k]) continue;
@@ -566,10 +672,10 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
     }
     const WalkStream W{J.list, J.composite, J.nmin, J.fac, J.fcap, J.ctrl};
     switch (walk_mode(J.modes)) {
-        case 0: walk_steps<0, K>(ra, rb, L, it.j0, it.j1, W); break;
-        case 1: walk_steps<1, K>(ra, rb, L, it.j0, it.j1, W); break;
-        case 2: walk_steps<2, K>(ra, rb, L, it.j0, it.j1, W); break;
-        default: walk_steps<3, K>(ra, rb, L, it.j0, it.j1, W); break;
+        case 0: walk_range<0, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
+        case 1: walk_range<1, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
+        case 2: walk_range<2, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
+        default: walk_range<3, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
     }
     uint64_t sum = 0;
 #pragma unroll
@@ -748,10 +854,10 @@ int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list
                 const uint32_t* d_comp, uint32_t nbase, const uint64_t* d_p, const uint32_t* d_v,
                 uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
                 pfc_factor* d_fac, uint32_t fcap, uint32_t cap, uint32_t ppl, uint32_t block,
-                hipStream_t st) {
+                uint32_t bmax, hipStream_t st) {
     if (j1 <= j0 || cap == 0) return PFC_OK;
     WalkArgs A;
-    A.j0 = j0; A.j1 = j1; A.list = d_list; A.composite = d_comp; A.nbase = nbase;
+    A.j0 = j0; A.j1 = j1; A.bmax = bmax; A.list = d_list; A.composite = d_comp; A.nbase = nbase;
     A.P = d_p; A.V = d_v; A.RA = d_ra; A.RB = d_rb; A.ctrl = d_ctrl;
     A.fac = d_fac; A.fcap = fcap; A.cap = cap;
     if (block != 64 && block != 128 && block != 256) block = 256;
@@ -901,8 +1007,11 @@ int pfc_stage_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_l
     pfc::g_err[0] = 0;
     if (!d_p || !d_v || !d_res_a || !d_ctrl || (!d_fac && fcap))
         return pfc::set_error(PFC_ERR_INVALID, "walk: null pointer");
+    // The multiplier of step j is j except in P mode, whose list this call
+    // cannot bound without reading it: P steps are then each tested alone.
+    const uint32_t bmax = (modes == PFC_MODE_PRIMORIAL || j1 <= j0) ? 0 : j1 - 1;
     return pfc::launch_walk(modes, j0, j1, d_list, d_composite, nbase, d_p, d_v, d_res_a,
-                            d_res_b, d_ctrl, d_fac, fcap, cap, 4, 256, (hipStream_t)stream);
+                            d_res_b, d_ctrl, d_fac, fcap, cap, 4, 256, bmax, (hipStream_t)stream);
 }
 
 int pfc_stage_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint32_t* d_v,

@@ -543,3 +543,57 @@ def test_factor_buffer_overflow_splits_segments(orc, dev):
     with pytest.raises(PK.PfcError) as e:
         PK.run_sieve(F, 103, 104, 101, 103, device=dev, factor_capacity=1)   # 101!-1 and 102!+1
     assert e.value.code == -4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("modes", [F, Cm, F | Cm])
+def test_walk_launches_end_at_hits(orc, dev, modes):
+    """The walk tests every second step of a chain when all p exceed twice the
+    largest multiplier, so a launch may end on an untested step.  Here every
+    launch ends at a hit step or right after one (p in [1e5, 3e5), n below
+    3001, so pairs are taken); factor records and checksum against the
+    oracle."""
+    import sympy
+    import pfcsieve_amd._native as Nt
+    from pfcsieve_amd.sieve import DeviceBuffer
+    L = Nt.lib()
+    nmin, nmax = 101, 3001
+    lo, hi = 10**5, 3 * 10**5
+    ref = orc.run(modes, lo, hi, nmin, nmax)
+    hits = sorted(set(abs(int(r["nc"])) for r in ref.factors))
+    assert len(hits) > 100
+    cuts = sorted({nmin, nmax} | {n for h in hits for n in (h, h + 1) if nmin < n < nmax})
+    cap = 1 << 15
+    two = modes == (F | Cm)
+    bufs = {k: DeviceBuffer(dev, cap * w) for k, w in [("p", 8), ("v", 4), ("one", 8), ("ra", 8), ("rb", 8)]}
+    ctrl = DeviceBuffer(dev, 32)
+    ctrl.zero()
+    fcap = 1 << 14
+    fac = DeviceBuffer(dev, 16 * fcap)
+    Nt.check(L.pfc_stage_generate(lo, hi, bufs["p"].ptr, cap, ctrl.ptr, None))
+    ta, keep_a = _table(dev, 0 if modes & F else 2, nmin - 1)
+    tb, keep_b = _table(dev, 2, nmin - 1)
+    Nt.check(L.pfc_stage_setup(bufs["p"].ptr, ctrl.ptr, bufs["v"].ptr, bufs["one"].ptr, bufs["ra"].ptr,
+                               bufs["rb"].ptr if two else None, C.byref(ta), C.byref(tb) if two else None,
+                               cap, None))
+    comp = np.zeros((nmax - nmin) // 32 + 2, np.uint32)
+    isp = set(int(x) for x in orc.primes(nmin, nmax - 1))
+    for k in range(nmax - nmin):
+        if nmin + k not in isp:
+            comp[k >> 5] |= np.uint32(1 << (k & 31))
+    dcomp = DeviceBuffer(dev, comp.nbytes)
+    dcomp.upload(comp.tobytes())
+    for j0, j1 in zip(cuts[:-1], cuts[1:]):
+        Nt.check(L.pfc_stage_walk(modes, j0, j1, None, dcomp.ptr if modes & Cm else None, nmin, bufs["p"].ptr,
+                                  bufs["v"].ptr, bufs["ra"].ptr, bufs["rb"].ptr if two else None,
+                                  ctrl.ptr, fac.ptr, fcap, cap, None))
+    Nt.check(L.pfc_stage_check(modes, nmax, bufs["p"].ptr, bufs["v"].ptr, bufs["one"].ptr, bufs["ra"].ptr,
+                               bufs["rb"].ptr if two else None, ctrl.ptr, cap, None))
+    Nt.check(L.pfc_device_synchronize(dev.handle))
+    c = np.frombuffer(ctrl.download(), np.uint64)
+    u = np.frombuffer(ctrl.download(), np.uint32)
+    assert int(c[2]) == ref.dev_checksum and int(c[3]) == ref.primecount == int(u[0])
+    assert int(u[1]) == ref.raw_factors and ref.raw_factors <= fcap
+    raw = np.frombuffer(fac.download(16 * int(u[1])), dtype=orc.FACTOR_DTYPE)
+    got = sorted((int(r["p"]), int(r["nc"]), int(r["type"])) for r in raw if sympy.isprime(int(r["p"])))
+    assert got == sorted((int(r["p"]), int(r["nc"]), int(r["type"])) for r in ref.factors)
