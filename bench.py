@@ -26,9 +26,9 @@ sys.path.insert(0, ROOT)
 W_MUL = 19  # INT32 multiply instructions per update in the reference formulation
 # HBM bytes per candidate per walk launch, from the FETCH_SIZE (x2.000 copy-kernel
 # calibration) and WRITE_SIZE counters of the walk on this workload
-# (profiles/k_walk_fc_1e12_pmc.md): p, v, one' and both residues read, both
-# residues written.
-WALK_FETCH_PER_CAND = 36.15
+# (profiles/k_walk_fc_1e12_pmc.md): p, v and both residues read, both residues
+# written.
+WALK_FETCH_PER_CAND = 28.77
 WALK_WRITE_PER_CAND = 16.00
 
 
@@ -229,14 +229,14 @@ def other_configs(dev):
     # segment's device time minus its walk launches (generate and check are
     # ~3 ms of it).
     nchunks, nbits = P.setup_table_size(0, 10**6 - 1)
-    products = nchunks + nbits - 1 + 1           # chunk products, squarings, the final product by R
+    products = nchunks + nbits - 1               # chunk products and squarings
     j = P.Job(dev, 1, 10**15, 10**15 + 2 * 10**9, 10**6, 10**6 + 256)
     j.run(1)
     s = j.stats()
     j.destroy()
     setup_s = (s.device_ms - s.walk_ms) * 1e-3
     out["setup_nmin_1e6"] = {"workload": "-! n [1e6,1e6+256) p from 1e15, one p-segment: start residues "
-                                         "Mont(999999!) per candidate",
+                                         "999999! mod p per candidate",
                              "candidates": s.primecount, "modular_products_per_candidate": products,
                              "setup_s": setup_s, "candidates_per_s": s.primecount / setup_s,
                              "products_per_s": s.primecount * products / setup_s}
