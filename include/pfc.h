@@ -225,7 +225,7 @@ int pfc_stage_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pf
                        void* stream);
 
 /* Host-built start-residue tables (replacing cl_sieve.cpp:842-1326): chunks
- * are products of primes below 2^64, grouped by exponent bit, high bit first. */
+ * are products of primes below 2^63, grouped by exponent bit, high bit first. */
 typedef struct pfc_setup_table {
     const uint64_t* chunks;    /* device */
     const uint32_t* groups;    /* device: nbits+1 offsets into chunks, group b = bit b */

@@ -135,12 +135,14 @@ PFC_HD uint64_t addmod(uint64_t a, uint64_t b, const Modulus& m) {
     return (r < a || r >= d) ? r - d : r;
 }
 
-// (a * b) mod d for a < d and any 64-bit b (b is a plain integer): the
-// 128-bit product <t3,t2,t1,t0>, then two 3-by-2 steps, <t3,t2,t1> mod d and
-// <r,t0> mod d (both satisfy the <u2,u1> < d precondition because a < d and r
-// is canonical).  On the device the rare final corrections are taken as
-// wavefront-uniform branches.
-PFC_HD uint64_t mulmod64(uint64_t a, uint64_t b, const Modulus& m) {
+// (a * b) mod d up to one d, for plain integers a, b with a * b < d * 2^64
+// (a < d, or b < 2^63 and any a): the 128-bit product <t3,t2,t1,t0>, then
+// two 3-by-2 steps, <t3,t2,t1> mod d and <r,t0> mod d (the first step's
+// remainder is corrected so that it satisfies the second's <u2,u1> < d
+// precondition).  Returns x == r or r + d for the remainder r and sets *fix
+// when x >= d, like rem3by2_nofix.  On the device the rare correction of the
+// first step is a wavefront-uniform branch.
+PFC_HD uint64_t mulmod64_nofix(uint64_t a, uint64_t b, const Modulus& m, bool* fix) {
     const uint64_t d = mod_d(m), nd = 0ull - d;
     const uint32_t k1 = walk_k1(m.d1, m.d0), k0 = walk_k0(m.d0);
     const uint64_t a0 = (uint32_t)a, a1 = a >> 32, b0 = (uint32_t)b, b1 = b >> 32;
@@ -148,17 +150,23 @@ PFC_HD uint64_t mulmod64(uint64_t a, uint64_t b, const Modulus& m) {
     const uint64_t p01 = a0 * b1 + (p00 >> 32);
     const uint64_t p10 = a1 * b0 + (uint32_t)p01;
     const uint64_t p11 = a1 * b1 + (p01 >> 32) + (p10 >> 32);
-    bool f1, f2;
+    bool f1;
     uint64_t r = rem3by2_nofix(p11, (uint32_t)p10, k1, k0, nd, d, m.v, &f1);
 #if defined(__HIP_DEVICE_COMPILE__)
     if (__builtin_expect(__ballot(f1) != 0, 0))
 #endif
         if (r >= d) r -= d;
-    r = rem3by2_nofix(r, (uint32_t)p00, k1, k0, nd, d, m.v, &f2);
+    return rem3by2_nofix(r, (uint32_t)p00, k1, k0, nd, d, m.v, fix);
+}
+
+// (a * b) mod d for a < d and any 64-bit b (b is a plain integer).
+PFC_HD uint64_t mulmod64(uint64_t a, uint64_t b, const Modulus& m) {
+    bool f;
+    uint64_t r = mulmod64_nofix(a, b, m, &f);
 #if defined(__HIP_DEVICE_COMPILE__)
-    if (__builtin_expect(__ballot(f2) != 0, 0))
+    if (__builtin_expect(__ballot(f) != 0, 0))
 #endif
-        if (r >= d) r -= d;
+        if (f) r -= mod_d(m);
     return r;
 }
 
@@ -294,16 +302,29 @@ PFC_HD bool is_prime_u32(uint32_t n) {
 
 // Start residue (X mod p) << s of X = prod over groups b (high first) of
 // (prod of chunks in group b)^(2^b), by Horner's rule in the shifted
-// representation (every product by mulmod64, chunks as plain integers).
+// representation, chunks as plain integers below 2^63: the chunk products
+// (mulmod64_nofix) accept the unreduced accumulator, which is reduced only
+// before each squaring and at the end.  With first/stride, only the chunks
+// first, first + stride, ... of each group are taken: the product over
+// first = 0 .. stride - 1 of those partial values is the full one, which
+// lets a workgroup share the work.
 template <class Chunks, class Groups>
-PFC_HD uint64_t start_residue(Chunks chunks, Groups groups, uint32_t nbits, const Modulus& m) {
+PFC_HD uint64_t start_residue(Chunks chunks, Groups groups, uint32_t nbits, const Modulus& m,
+                              uint32_t first = 0, uint32_t stride = 1) {
+    const uint64_t d = mod_d(m);
     uint64_t acc = 1ull << m.s;                                     // 1, shifted
     for (int b = (int)nbits - 1; b >= 0; --b) {
-        if (b != (int)nbits - 1) acc = mulmod64(acc, acc >> m.s, m);
+        if (b != (int)nbits - 1) {
+            if (acc >= d) acc -= d;
+            acc = mulmod64(acc, acc >> m.s, m);
+        }
         const uint32_t g0 = groups[b], g1 = groups[b + 1];
-        for (uint32_t i = g0; i < g1; ++i) acc = mulmod64(acc, chunks[i], m);
+        for (uint32_t i = g0 + first; i < g1; i += stride) {
+            bool fix;
+            acc = mulmod64_nofix(acc, chunks[i], m, &fix);
+        }
     }
-    return acc;
+    return acc >= d ? acc - d : acc;
 }
 
 // Montgomery form (x * 2^64 mod p) << s of a canonical shifted residue x,

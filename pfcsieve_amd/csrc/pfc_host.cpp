@@ -70,13 +70,16 @@ std::vector<uint32_t> primes_in(uint64_t lo, uint64_t hi) {
 
 // Setup table (include/pfc.h pfc_setup_table): exponents of every prime r <= N
 // in X(N) = N! (kind 0), N# (kind 1) or N!/N# (kind 2); group b holds the
-// primes with bit b of the exponent set, multiplied into chunks below 2^64.
+// primes with bit b of the exponent set, multiplied into chunks below 2^63
+// (so that a chunk product accepts an unreduced accumulator).
 struct HostTable {
     std::vector<uint64_t> chunks;
     std::vector<uint32_t> groups;  // nbits + 1 offsets
     uint32_t nbits = 0;
     uint32_t kind = 0, N = 0;      // X(N) = N!, N# or N!/N#
 };
+
+constexpr u128 kChunkLimit = (u128)1 << 63;
 
 HostTable build_table(uint32_t kind, uint32_t N) {
     HostTable T;
@@ -101,16 +104,19 @@ HostTable build_table(uint32_t kind, uint32_t N) {
     // fit, then the smallest remaining ones as filler (about 4% fewer chunks
     // than filling in increasing order; the group's product is unchanged).
     std::vector<uint32_t> g;
+    // Exponents fall with the prime (e(r) <= N / (r - 1)), so bit b >= 1 can be
+    // set only for r <= N / 2^b + 1: each group scans the primes up to there.
     for (uint32_t b = 0; b < T.nbits; ++b) {
         T.groups[b] = (uint32_t)T.chunks.size();
         g.clear();
-        for (size_t i = 0; i < pr.size(); ++i)
+        const uint64_t rmax = b ? (uint64_t)N / (1ull << b) + 1 : UINT64_MAX;
+        for (size_t i = 0; i < pr.size() && pr[i] <= rmax; ++i)
             if ((ex[i] >> b) & 1u) g.push_back(pr[i]);          // ascending
         size_t lo = 0, hi = g.size();
         while (lo < hi) {
             uint64_t acc = g[--hi];
-            while (lo < hi && (u128)acc * g[hi - 1] <= 0xFFFFFFFFFFFFFFFFull) acc *= g[--hi];
-            while (lo < hi && (u128)acc * g[lo] <= 0xFFFFFFFFFFFFFFFFull) acc *= g[lo++];
+            while (lo < hi && (u128)acc * g[hi - 1] < kChunkLimit) acc *= g[--hi];
+            while (lo < hi && (u128)acc * g[lo] < kChunkLimit) acc *= g[lo++];
             T.chunks.push_back(acc);
         }
     }

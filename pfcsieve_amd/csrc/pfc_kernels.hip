@@ -212,16 +212,34 @@ __global__ __launch_bounds__(256) void k_verify_slow(uint32_t kind, uint32_t N, 
     if (threadIdx.x == 0) work[2 + blockIdx.x] = sh[0];
 }
 
-__global__ __launch_bounds__(64) void k_verify_finish(const uint64_t* __restrict__ chunks,
-                                                      const uint32_t* __restrict__ groups, uint32_t nbits,
-                                                      uint64_t* __restrict__ work) {
-    if (threadIdx.x != 0) return;
-    const Modulus m = pfc::make_modulus(kTestPrime);
-    uint64_t x = 1;
-    for (uint32_t i = 0; i < PFC_VERIFY_BLOCKS; ++i) x = pfc::mulmod64(x, work[2 + i], m);
-    const uint64_t one = pfc::mont_one(m);
-    work[0] = pfc::mulmod64(one, x, m);                  // X * R mod p
-    work[1] = pfc::to_mont(horner_start(chunks, groups, nbits, m), one, m);
+// One workgroup of kVerifyLanes lanes: lane t multiplies the slow product's
+// block partial t into its share, and evaluates the table's chunks t,
+// t + kVerifyLanes, ... of every group through the set-up's Horner code; the
+// shares of both values are multiplied together over the workgroup.
+constexpr uint32_t kVerifyLanes = 256;
+static_assert(PFC_VERIFY_BLOCKS == kVerifyLanes, "one slow-product partial per lane");
+
+__global__ __launch_bounds__(kVerifyLanes) void k_verify_finish(const uint64_t* __restrict__ chunks,
+                                                                const uint32_t* __restrict__ groups,
+                                                                uint32_t nbits, uint64_t* __restrict__ work) {
+    const Modulus m = pfc::make_modulus(kTestPrime);     // s = 0: residues are plain
+    const uint32_t t = threadIdx.x;
+    __shared__ uint64_t sx[kVerifyLanes], st[kVerifyLanes];
+    sx[t] = work[2 + t];
+    st[t] = pfc::start_residue((ConstU64)chunks, (ConstU32)groups, nbits, m, t, kVerifyLanes);
+    __syncthreads();
+    for (uint32_t h = kVerifyLanes / 2; h > 0; h >>= 1) {
+        if (t < h) {
+            sx[t] = pfc::mulmod64(sx[t], sx[t + h], m);
+            st[t] = pfc::mulmod64(st[t], st[t + h], m);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const uint64_t one = pfc::mont_one(m);
+        work[0] = pfc::to_mont(sx[0], one, m);           // X * R mod p
+        work[1] = pfc::to_mont(st[0], one, m);
+    }
 }
 
 // ------------------------------------------------------------------ walk
@@ -918,7 +936,7 @@ int launch_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uin
     hipLaunchKernelGGL(k_verify_slow, dim3(PFC_VERIFY_BLOCKS), dim3(256), 0, st, kind, N, d_work);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "k_verify_slow launch");
-    hipLaunchKernelGGL(k_verify_finish, dim3(1), dim3(64), 0, st, t->chunks, t->groups, t->nbits,
+    hipLaunchKernelGGL(k_verify_finish, dim3(1), dim3(kVerifyLanes), 0, st, t->chunks, t->groups, t->nbits,
                        d_work);
     e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_verify_finish launch");

@@ -15,6 +15,13 @@ uint64_t t_mulmod32(uint64_t p, uint64_t a_shifted, uint32_t n) {
 uint64_t t_mulmod64(uint64_t p, uint64_t a_shifted, uint64_t b) {
     return pfc::mulmod64(a_shifted, b, pfc::make_modulus(p));
 }
+// mulmod64_nofix: *fix set when the result is unreduced.
+uint64_t t_mulmod64_nofix(uint64_t p, uint64_t a_shifted, uint64_t b, int* fix) {
+    bool f;
+    const uint64_t r = pfc::mulmod64_nofix(a_shifted, b, pfc::make_modulus(p), &f);
+    *fix = f;
+    return r;
+}
 uint64_t t_sqrmod(uint64_t p, uint64_t a_shifted) { return pfc::sqrmod(a_shifted, pfc::make_modulus(p)); }
 uint64_t t_mont_one(uint64_t p) { return pfc::mont_one(pfc::make_modulus(p)); }
 int t_modulus_ok(uint64_t p) { return pfc::modulus_ok(p, pfc::make_modulus(p)); }

@@ -174,7 +174,7 @@ def _table(dev, kind, N, corrupt=False):
     Nt.check(L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb)))
     if corrupt:
         assert nch.value > 0
-        ch[0] = ch[0] + 2 if ch[0] < 2**63 else ch[0] - 2
+        ch[0] = ch[0] + 2 if ch[0] < 2**62 else ch[0] - 2
     dc = DeviceBuffer(dev, 8 * max(nch.value, 1))
     dc.upload(bytes(ch))
     dg = DeviceBuffer(dev, 4 * (nb.value + 1))

@@ -132,7 +132,7 @@ def test_setup_table_reconstructs_start_value(L, kind, N):
     prim = sympy.primorial(N, nth=False)
     expect = [math.factorial(N), prim, math.factorial(N) // prim][kind]
     assert x == expect
-    assert all(1 < c < 2**64 for c in ch)
+    assert all(1 < c < 2**63 for c in ch)
 
 
 # ----------------------------------------------------------------- device arithmetic on the host
@@ -167,6 +167,8 @@ def A():
     L.t_walk_step.argtypes = [u64, u64, u32, C.POINTER(u64), C.POINTER(C.c_int)]
     L.t_walk_step.restype = u64
     L.t_walk_attention.argtypes = [u64, u64]
+    L.t_mulmod64_nofix.argtypes = [u64, u64, u64, C.POINTER(C.c_int)]
+    L.t_mulmod64_nofix.restype = u64
     return L
 
 
@@ -201,6 +203,26 @@ def test_mulmod_against_bigint(A):
             assert A.t_mulmod64(p, a << s, b) == (a * b % p) << s
             assert A.t_sqrmod(p, a << s) == (a * a % p) << s
             assert A.t_to_mont(p, a << s) == (a * R % p) << s
+
+
+def test_mulmod64_unreduced_input(A):
+    """The start-residue chunk product: for b < 2^63 any 64-bit input is
+    accepted, unreduced ones (x << s) + d included; the result is a * b mod p,
+    shifted, or that plus d with the fix flag set."""
+    rng = random.Random(13)
+    fix = C.c_int()
+    for p in _primes_mix(rng, 2000):
+        s = 64 - p.bit_length()
+        d = p << s
+        for _ in range(4):
+            x = rng.randrange(p)
+            a = x << s
+            if a + d < R and rng.random() < 0.5:
+                a += d
+            for b in [1, 2**63 - 1, rng.randrange(2**63), rng.randrange(2**32)]:
+                got = A.t_mulmod64_nofix(p, a, b, C.byref(fix))
+                want = (x * b % p) << s
+                assert got in (want, want + d) and bool(fix.value) == (got >= d), (p, a, b)
 
 
 def test_walk_attention_bounds(A):
