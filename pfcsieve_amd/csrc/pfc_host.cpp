@@ -950,13 +950,18 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
 //   1. jobs that need a new p-segment get one (clear control block, generate
 //      candidates) -- all jobs' generation launches are queued back to back;
 //   2. the candidates of every active segment are cut into work items of
-//      64*K candidates x up to `max_steps` multiplier steps, ordered by
-//      decreasing cost (longest first), and ONE launch of the persistent
-//      k_batch kernel drains them from a device queue;
+//      64*K candidates x up to `max_steps` multiplier steps, described by one
+//      range per segment and ordered by decreasing cost (longest first), and
+//      ONE launch of the persistent k_batch kernel drains them from a device
+//      queue (a counter; each item finds its segment by binary search);
 //   3. segments whose walk finished (their last items carried the checksum
 //      fold) are drained to the host like pfc_job_run does; finished jobs are
 //      returned to the caller.
 // Jobs can be added between rounds; their state stays resident on the device.
+
+// Work items per batch launch at most (the queue counter is 32-bit); segments
+// beyond it wait for the next round.
+constexpr uint64_t kMaxBatchItems = 1ull << 31;
 
 struct pfc_batcher {
     pfc_device* dev = nullptr;
@@ -972,8 +977,8 @@ struct pfc_batcher {
     std::vector<Slot> slots;
     pfc_batch_job* d_jobs = nullptr;
     size_t d_jobs_cap = 0;
-    pfc_batch_item* d_items = nullptr;
-    size_t d_items_cap = 0;
+    pfc_batch_range* d_ranges = nullptr;
+    size_t d_ranges_cap = 0;
     uint32_t* d_head = nullptr;
     pfc_ctrl* h_ctrl = nullptr;       // pinned, one per slot
     size_t h_ctrl_cap = 0;
@@ -1138,16 +1143,19 @@ extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** d
             s.stage = 1;
         }
     }
-    // 2. work items
+    // 2. work items: one range of items per segment (64*K candidates each),
+    // ranges ordered by their items' cost, longest first, so that the queue
+    // ends on short items.
     std::vector<pfc_batch_job> jobs(ns);
-    struct Tmp { pfc_batch_item it; double cost; };
-    std::vector<Tmp> items;
+    struct Tmp { pfc_batch_range r; uint32_t count; double cost; };
+    std::vector<Tmp> ranges;
     uint32_t K;
     {
         std::lock_guard<std::mutex> g(b->dev->mu);
         K = b->dev->tuning.primes_per_lane;
     }
     std::vector<uint8_t> finishing(ns, 0);
+    uint64_t nitems = 0;
     for (size_t i = 0; i < ns; ++i) {
         pfc_batcher::Slot& s = b->slots[i];
         jobs[i] = batch_view(s.job);
@@ -1161,28 +1169,36 @@ extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** d
             s.jcur = j->jend;
             continue;
         }
+        const uint32_t count = (uint32_t)((s.ncand + 64ull * K - 1) / (64ull * K));
+        if (nitems + count > kMaxBatchItems) continue;   // left for the next round
+        nitems += count;
+        // Relative cost of one item: walk steps (the F|C walk takes ~1.9 steps
+        // per n) plus the start-residue chunk products, each about two steps.
         const double per = (double)(j1 - j0) * ((j->d_rb) ? 1.9 : 1.0) +
-                           ((flags & 1u) ? 2.2 * (double)(j->ta.chunks.size() + j->tb.chunks.size()) : 0.0);
-        const uint32_t bmax = walk_bound(j, j1);
-        for (uint32_t f = 0; f < s.ncand; f += 64 * K)
-            items.push_back({{(uint32_t)i, f, j0, j1, flags, bmax, {0, 0}}, per});
+                           ((flags & 1u) ? 2.0 * (double)(j->ta.chunks.size() + j->tb.chunks.size()) : 0.0);
+        ranges.push_back({{(uint32_t)i, j0, j1, flags, walk_bound(j, j1), 0, {0, 0}}, count, per});
         s.jcur = j1;
         if (j1 == j->jend) finishing[i] = 1;
     }
-    std::stable_sort(items.begin(), items.end(), [](const Tmp& a, const Tmp& c) { return a.cost > c.cost; });
-    if (!items.empty()) {
+    std::stable_sort(ranges.begin(), ranges.end(), [](const Tmp& a, const Tmp& c) { return a.cost > c.cost; });
+    if (!ranges.empty()) {
         rc = ensure_device(&b->d_jobs, &b->d_jobs_cap, ns);
-        if (!rc) rc = ensure_device(&b->d_items, &b->d_items_cap, items.size());
+        if (!rc) rc = ensure_device(&b->d_ranges, &b->d_ranges_cap, ranges.size());
         if (rc) return rc;
-        std::vector<pfc_batch_item> flat(items.size());
-        for (size_t k = 0; k < items.size(); ++k) flat[k] = items[k].it;
+        std::vector<pfc_batch_range> flat(ranges.size());
+        uint32_t end = 0;
+        for (size_t k = 0; k < ranges.size(); ++k) {
+            end += ranges[k].count;
+            flat[k] = ranges[k].r;
+            flat[k].end = end;
+        }
         hipError_t e = hipMemcpyAsync(b->d_jobs, jobs.data(), ns * sizeof(pfc_batch_job), hipMemcpyHostToDevice, st);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(b->d_items, flat.data(), flat.size() * sizeof(pfc_batch_item), hipMemcpyHostToDevice, st);
+            e = hipMemcpyAsync(b->d_ranges, flat.data(), flat.size() * sizeof(pfc_batch_range), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemsetAsync(b->d_head, 0, sizeof(uint32_t), st);
         if (e != hipSuccess) return hip_fail(e, "batcher upload");
-        const uint32_t nblocks = (uint32_t)std::min<size_t>(items.size(), (size_t)b->dev->prop.multiProcessorCount * 32);
-        rc = pfc::launch_batch(b->d_jobs, b->d_items, (uint32_t)items.size(), b->d_head, K, nblocks, st);
+        const uint32_t nblocks = (uint32_t)std::min<uint64_t>(nitems, (uint64_t)b->dev->prop.multiProcessorCount * 32);
+        rc = pfc::launch_batch(b->d_jobs, b->d_ranges, (uint32_t)flat.size(), b->d_head, K, nblocks, st);
         if (rc) return rc;
     }
     // 3. drain finished segments
@@ -1223,7 +1239,7 @@ extern "C" int pfc_batcher_destroy(pfc_batcher* b) {
     hipSetDevice(b->dev->ordinal);
     (void)hipStreamSynchronize(b->stream);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
-    if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_ranges) (void)hipFree(b->d_ranges);
     if (b->d_head) (void)hipFree(b->d_head);
     if (b->h_ctrl) (void)hipHostFree(b->h_ctrl);
     (void)hipStreamDestroy(b->stream);

@@ -35,6 +35,14 @@ struct pfc_batch_item {
     uint32_t job, first, j0, j1, flags, bmax, pad[2];
 };
 
+// The work items of one job segment in one launch: items [start, end) of the
+// launch's queue, where start is the previous range's end (0 for the first);
+// item start + k covers candidates from k * 64*K.  Ranges are stored in queue
+// order with increasing ends.
+struct pfc_batch_range {
+    uint32_t job, j0, j1, flags, bmax, end, pad[2];
+};
+
 namespace pfc {
 
 extern thread_local char g_err[512];
@@ -57,7 +65,7 @@ int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list
 int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint32_t* d_v,
                  const uint64_t* d_one, const uint64_t* d_ra, const uint64_t* d_rb,
                  pfc_ctrl* d_ctrl, uint32_t cap, hipStream_t st);
-int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_item* d_items, uint32_t nitems,
+int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, uint32_t nranges,
                  uint32_t* d_head, uint32_t ppl, uint32_t nblocks, hipStream_t st);
 int launch_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_work,
                         hipStream_t st);

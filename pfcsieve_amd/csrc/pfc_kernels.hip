@@ -723,17 +723,34 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
     }
 }
 
+// The queue is a counter over the launch's items; an item's job segment is
+// found by a wavefront-uniform binary search over the ranges' ends.
 template <int K>
 __global__ __launch_bounds__(64) void k_batch(const pfc_batch_job* __restrict__ jobs,
-                                              const pfc_batch_item* __restrict__ items,
-                                              uint32_t nitems, uint32_t* head) {
+                                              const pfc_batch_range* __restrict__ ranges,
+                                              uint32_t nranges, uint32_t* head) {
+    const uint32_t nitems = ranges[nranges - 1].end;
     for (;;) {
         uint32_t i = 0;
         if (__lane_id() == 0) i = atomicAdd(head, 1u);
         i = __builtin_amdgcn_readfirstlane(__shfl(i, 0));
         if (i >= nitems) break;
-        const pfc_batch_item it = items[i];
-        const pfc_batch_job J = jobs[it.job];
+        uint32_t lo = 0, hi = nranges - 1;      // first range whose end exceeds i
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (ranges[mid].end > i) hi = mid;
+            else lo = mid + 1;
+        }
+        const pfc_batch_range R = ranges[lo];
+        const uint32_t start = lo ? ranges[lo - 1].end : 0u;
+        pfc_batch_item it;
+        it.job = R.job;
+        it.first = (i - start) * (64u * K);
+        it.j0 = R.j0;
+        it.j1 = R.j1;
+        it.flags = R.flags;
+        it.bmax = R.bmax;
+        const pfc_batch_job J = jobs[R.job];
         batch_item<K>(J, it);
     }
 }
@@ -919,13 +936,13 @@ int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint3
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_check launch");
 }
 
-int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_item* d_items, uint32_t nitems,
+int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, uint32_t nranges,
                  uint32_t* d_head, uint32_t ppl, uint32_t nblocks, hipStream_t st) {
-    if (nitems == 0) return PFC_OK;
+    if (nranges == 0) return PFC_OK;
     if (nblocks == 0) nblocks = 1;
-    if (ppl == 1) hipLaunchKernelGGL(k_batch<1>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_items, nitems, d_head);
-    else if (ppl == 2) hipLaunchKernelGGL(k_batch<2>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_items, nitems, d_head);
-    else if (ppl == 4) hipLaunchKernelGGL(k_batch<4>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_items, nitems, d_head);
+    if (ppl == 1) hipLaunchKernelGGL(k_batch<1>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_ranges, nranges, d_head);
+    else if (ppl == 2) hipLaunchKernelGGL(k_batch<2>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_ranges, nranges, d_head);
+    else if (ppl == 4) hipLaunchKernelGGL(k_batch<4>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_ranges, nranges, d_head);
     else return set_error(PFC_ERR_INVALID, "batch: primes_per_lane must be 1, 2 or 4");
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_batch launch");
