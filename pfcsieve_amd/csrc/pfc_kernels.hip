@@ -433,21 +433,6 @@ __device__ __forceinline__ void pair(uint64_t (&r)[K], uint32_t m, uint32_t n, i
     if (step<K>(r, n, L)) settle<K>(r, m, n, type, L, W.fac, W.fcap, W.ctrl);
 }
 
-// A step of a chain whose steps come one at a time (composite multipliers):
-// with pairs, *pend holds the untested step before n (0 = none), uniformly.
-template <int K, bool PAIRED>
-__device__ __forceinline__ void chain_step(uint64_t (&r)[K], uint32_t n, uint32_t* pend, int32_t type,
-                                           const Lanes<K>& L, const WalkStream& W) {
-    mul_step<K>(r, n, L);
-    if (PAIRED) untested<K>(r);
-    if (!PAIRED || *pend) {
-        if (attention<K>(r, L)) settle<K>(r, PAIRED ? *pend : 0, n, type, L, W.fac, W.fcap, W.ctrl);
-        *pend = 0;
-    } else {
-        *pend = n;
-    }
-}
-
 template <int MODE, int K, bool PAIRED>
 __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K], const Lanes<K>& L,
                                            uint32_t j0, uint32_t j1, const WalkStream& W) {
@@ -508,13 +493,12 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
         for (; j < j1; ++j) single<K>(ra, __builtin_amdgcn_readfirstlane(list[j]), PFC_PRIMORIAL, L, W);
     } else {
         // F|C (MODE 2) and C (MODE 3): composite flags come from the bitmap,
-        // one 32-bit word per 32 steps.  A whole word inside [j0, j1) takes an
-        // unrolled path whose flag tests use constant bit indices (one scalar
-        // bit-compare and branch per n, no shifts or loop counter); partial
-        // words at either end of the range take the generic loop.  The
-        // composite chain (res_b in F|C, res_a in C) is paired across words
-        // through `pend`; the factorial chain of F|C pairs n, n + 1 inside a
-        // word.
+        // one 32-bit word per 32 steps, limited to [j0, j1).  The factorial
+        // chain of F|C takes every n of the word (16 fixed pairs for a whole
+        // word); the composite chain (res_b in F|C, res_a in C) takes the set
+        // bits two at a time with a scalar bit scan, one branch per pair, and
+        // carries an odd one over to the next word in `pend`.  The chains are
+        // independent, so their order within a word does not matter.
         const ConstU32 comp = (ConstU32)W.composite;
         uint64_t (&rc)[K] = (MODE == 2) ? rb : ra;
         uint32_t pend = 0;
@@ -523,32 +507,48 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
             const uint32_t off = j - W.nbase;
             const uint32_t wbase = off & ~31u;
             const uint32_t nword = W.nbase + wbase;
-            const uint32_t word = __builtin_amdgcn_readfirstlane(comp[wbase >> 5]);
-            if ((off & 31u) == 0u && j1 - j >= 32u) {
-#pragma unroll
-                for (uint32_t t = 0; t < 32; ++t) {
-                    const uint32_t n = __builtin_amdgcn_readfirstlane(nword + t);
-                    if (MODE == 2) {
-                        if (!PAIRED) single<K>(ra, n, PFC_FACTORIAL, L, W);
-                        else if (t & 1u) {
-                            if (step<K>(ra, n, L)) settle<K>(ra, n - 1, n, PFC_FACTORIAL, L, W.fac, W.fcap, W.ctrl);
-                        } else {
-                            mul_step<K>(ra, n, L);
-                            untested<K>(ra);
-                        }
-                    }
-                    if ((word >> t) & 1u) chain_step<K, PAIRED>(rc, n, &pend, PFC_COMPOSITORIAL, L, W);
-                }
-                j = nword + 32u;
-                continue;
-            }
             const uint32_t jend = min(j1, nword + 32u);
-            for (; j < jend; ++j) {
-                const uint32_t n = __builtin_amdgcn_readfirstlane(j);
-                if (MODE == 2) single<K>(ra, n, PFC_FACTORIAL, L, W);
-                if ((word >> ((n - W.nbase) & 31u)) & 1u)
-                    chain_step<K, PAIRED>(rc, n, &pend, PFC_COMPOSITORIAL, L, W);
+            const uint32_t lo = j - nword, hi = jend - nword;          // bits [lo, hi) of the word
+            uint32_t word = __builtin_amdgcn_readfirstlane(comp[wbase >> 5]);
+            word &= (~0u << lo) & (hi >= 32u ? ~0u : (1u << hi) - 1u);
+            if (MODE == 2) {
+                if (PAIRED && lo == 0u && hi == 32u) {
+#pragma unroll
+                    for (uint32_t t = 0; t < 32; t += 2)
+                        pair<K>(ra, __builtin_amdgcn_readfirstlane(nword + t),
+                                __builtin_amdgcn_readfirstlane(nword + t + 1), PFC_FACTORIAL, L, W);
+                } else {
+                    for (uint32_t n = j; n < jend; ++n)
+                        single<K>(ra, __builtin_amdgcn_readfirstlane(n), PFC_FACTORIAL, L, W);
+                }
             }
+            if (PAIRED) {
+                if (pend && word) {
+                    const uint32_t n = nword + (uint32_t)__builtin_ctz(word);
+                    word &= word - 1u;
+                    if (step<K>(rc, n, L)) settle<K>(rc, pend, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
+                    pend = 0;
+                }
+                while (word & (word - 1u)) {                            // two or more left
+                    const uint32_t m = nword + (uint32_t)__builtin_ctz(word);
+                    word &= word - 1u;
+                    const uint32_t n = nword + (uint32_t)__builtin_ctz(word);
+                    word &= word - 1u;
+                    pair<K>(rc, m, n, PFC_COMPOSITORIAL, L, W);
+                }
+                if (word) {
+                    pend = nword + (uint32_t)__builtin_ctz(word);
+                    mul_step<K>(rc, pend, L);
+                    untested<K>(rc);
+                }
+            } else {
+                while (word) {
+                    const uint32_t n = nword + (uint32_t)__builtin_ctz(word);
+                    word &= word - 1u;
+                    single<K>(rc, n, PFC_COMPOSITORIAL, L, W);
+                }
+            }
+            j = jend;
         }
         // A last untested composite step is tested alone.
         if (PAIRED && pend) settle<K>(rc, 0, pend, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
