@@ -294,23 +294,44 @@ def job_result(job: Job, verify_threads: int = 2) -> SieveResult:
 
 
 def run_batch(descs, *, device: Optional[Device] = None, max_steps: int = 0,
-              verify_threads: int = 2, progress=None) -> List[SieveResult]:
+              verify_threads: int = 2, progress=None,
+              hbm_budget: Optional[int] = None) -> List[SieveResult]:
     """Runs job descriptions (modes, pmin, pmax, nmin, nmax) through one batcher.
-    progress: optional callable receiving one line per round."""
+
+    Jobs are admitted in order while the sum of their device footprints
+    (pfc_job_footprint) fits hbm_budget (default: 80% of the device's memory);
+    finished jobs are harvested and released between rounds and the next ones
+    admitted, so a list larger than the device runs in one pass.  Results come
+    back in the order of descs.  progress: optional callable receiving one
+    line per round."""
     dev = device or Device(0)
-    jobs = [Job(dev, *d) for d in descs]
+    budget = hbm_budget if hbm_budget is not None else int(dev.info()["hbm_bytes"] * 0.8)
+    sizes = [footprint(dev, *d) for d in descs]
+    for i, size in enumerate(sizes):
+        if size > budget:
+            raise ValueError(f"job {i} needs {size} bytes of device memory, over the budget of {budget}")
+    results: List[Optional[SieveResult]] = [None] * len(descs)
+    active = {}             # job handle -> (index, job)
+    resident, nxt, rounds = 0, 0, 0
     b = Batcher(dev)
     try:
-        for j in jobs:
-            b.add(j)
-        rounds = 0
-        while b.pending():
-            b.round(max_steps)
+        while nxt < len(descs) or active:
+            while nxt < len(descs) and resident + sizes[nxt] <= budget:
+                j = Job(dev, *descs[nxt])
+                b.add(j)
+                active[j.handle.value] = (nxt, j)
+                resident += sizes[nxt]
+                nxt += 1
+            for job in b.round(max_steps):
+                i, j = active.pop(job.handle.value)
+                results[i] = job_result(j, verify_threads)
+                resident -= sizes[i]
+                j.destroy()
             rounds += 1
             if progress:
-                progress(f"batch round {rounds}: {b.pending()} of {len(jobs)} jobs pending")
-        return [job_result(j, verify_threads) for j in jobs]
+                progress(f"batch round {rounds}: {len(descs) - nxt + len(active)} of {len(descs)} jobs pending")
+        return results
     finally:
         b.destroy()
-        for j in jobs:
+        for _, j in active.values():
             j.destroy()

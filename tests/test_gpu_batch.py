@@ -33,6 +33,24 @@ def test_batch_matches_oracle(orc, dev, max_steps):
         assert (g.checksum, g.primecount, g.factors) == (cs, pc, fac), job
 
 
+def test_batch_admits_by_memory_budget(orc, dev):
+    """A job list larger than the memory budget runs in one pass: with room
+    for about two jobs at a time, jobs are admitted as others finish, and
+    every result (returned in input order) is the oracle's."""
+    import pfcsieve_amd as PK
+    sizes = [PK.footprint(dev, *d) for d in MIX]
+    budget = max(sizes) + min(sizes)
+    seen = []
+    got = PK.run_batch(MIX, device=dev, max_steps=900, verify_threads=8, hbm_budget=budget,
+                       progress=seen.append)
+    assert len(seen) > len(MIX) // 2
+    for job, g in zip(MIX, got):
+        cs, pc, fac = _ref(orc, job)
+        assert (g.checksum, g.primecount, g.factors) == (cs, pc, fac), job
+    with pytest.raises(ValueError):
+        PK.run_batch(MIX[:1], device=dev, hbm_budget=sizes[0] - 1)
+
+
 def test_batch_multi_segment_jobs(orc, dev):
     import pfcsieve_amd as PK
     dev.set_tuning(seg_candidates=1000)
