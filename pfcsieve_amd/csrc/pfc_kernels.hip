@@ -5,14 +5,16 @@
 //                base-2 strong PRP), replacing kernels/getsegprimes.cl:464-526
 //                and kernels/addsmallprimes.cl:21-40.
 //   k_setup    : per-candidate reciprocal, R mod p and start residues
-//                Mont(X(nmin-1)), replacing kernels/setup.cl:49-288.
+//                X(nmin-1) mod p, replacing kernels/setup.cl:49-288.
 //   k_walk     : the residue walk, replacing kernels/iterate.cl:59-198.
-//   k_check    : the checksum fold, replacing kernels/check.cl:45-204.
+//   k_check    : the checksum fold (Montgomery terms), replacing
+//                kernels/check.cl:45-204.
+//   k_batch    : set-up, walk and fold of many jobs from one device queue.
 //
 // Device state of a segment is structure-of-arrays, one entry per candidate:
 // p (u64), v (u32 reciprocal), one (u64, R mod p shifted), res_a, res_b (u64
-// plain shifted residues X << s).  Candidates are appended in no particular order; every
-// later result (sums, sorted factor lists) is order independent.
+// plain shifted residues X << s).  Candidates are appended in no particular
+// order; every later result (sums, sorted factor lists) is order independent.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
