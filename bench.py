@@ -184,6 +184,28 @@ def verify_baseline(dev):
         t = time.perf_counter()
         check(lib().pfc_harvest(cp, n, threads, C.byref(cs), C.byref(kept)))
         out[f"threads_{threads}_s"] = time.perf_counter() - t
+    # The reference's own verify() (verifyprime.cpp:149, compiled from its
+    # source into oracle/_ref when the reference tree was present at build
+    # time) on the first records, one call per record on all host threads as
+    # getResults' OpenMP loop does, beside the library's harvest of the same
+    # records (which also sorts, filters by isPrime and sums the checksum).
+    import oracle as orc
+    if orc.ref_available():
+        from concurrent.futures import ThreadPoolExecutor
+        m = min(n, 4000)
+        threads = os.cpu_count() or 2
+        orc.ref_is_prime(3)                     # loads the library before the threads start
+        recs = [(int(f.p), abs(int(f.nc)), 1 if f.nc > 0 else -1, int(f.type)) for f in arr[:m]]
+        t = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            ok = list(ex.map(lambda r: orc.ref_verify(r[0], r[1], r[2], r[3], 1000000), recs))
+        t_ref = time.perf_counter() - t
+        cp = (type(arr[0]) * m)(*arr[:m])
+        cs, kept = C.c_uint64(0), C.c_size_t()
+        t = time.perf_counter()
+        check(lib().pfc_harvest(cp, m, threads, C.byref(cs), C.byref(kept)))
+        out["reference_verify"] = {"records": m, "threads": threads, "all_verified": all(ok),
+                                   "reference_s": t_ref, "library_harvest_s": time.perf_counter() - t}
     return out
 
 
