@@ -51,6 +51,21 @@ def test_batch_admits_by_memory_budget(orc, dev):
         PK.run_batch(MIX[:1], device=dev, hbm_budget=sizes[0] - 1)
 
 
+@pytest.mark.parametrize("ppl", [2, 4])
+def test_batch_wider_lanes(orc, dev, ppl):
+    """The batch kernel with 2 and 4 candidates per lane (k_batch<2>, <4>)
+    gives every job the oracle's result."""
+    import pfcsieve_amd as PK
+    dev.set_tuning(primes_per_lane=ppl)
+    try:
+        got = PK.run_batch(MIX, device=dev, max_steps=1100, verify_threads=8)
+    finally:
+        dev.set_tuning()
+    for job, g in zip(MIX, got):
+        cs, pc, fac = _ref(orc, job)
+        assert (g.checksum, g.primecount, g.factors) == (cs, pc, fac), (ppl, job)
+
+
 def test_batch_multi_segment_jobs(orc, dev):
     import pfcsieve_amd as PK
     dev.set_tuning(seg_candidates=1000)
