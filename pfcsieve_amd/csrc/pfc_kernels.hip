@@ -296,6 +296,12 @@ struct WalkArgs {
 // The untested result may stay unreduced: the 3-by-2 step accepts any 64-bit
 // input because n < 2^31.  The band passes a fraction 2 * bmax / p of the
 // residues.  Wavefronts with a smaller p test every step against +-1.
+//
+// Triple steps (P mode).  When every p of a wavefront exceeds 2^14 * bmax^2,
+// a residue that is +-1 after step a is +-(b * c) after steps b and c, with
+// b * c < p / 2^14: the band B = bmax^2 << s on every third step finds the
+// hits of all three (the slow path also compares with +-(b * c) << s) and
+// passes a fraction below 2^-13 of the residues.
 template <int K>
 struct Lanes {
     uint32_t k1[K], k0[K], v[K], s[K];    // k1, k0: pfc::walk_k1 / walk_k0 of d
@@ -316,22 +322,28 @@ __device__ __forceinline__ void lanes_from_state(Lanes<K>& L, uint64_t p, uint32
     L.thr[k] = L.d[k] - 2 * one - 1;
 }
 
-// True, uniformly, when the wavefront may take paired steps for multipliers
-// up to bmax (0 = unknown bound: never); then switches its test to the band.
+// Steps per test the wavefront may take for multipliers up to bmax (0 =
+// unknown bound: 1), uniformly: 3 (only where triples are allowed), 2 or 1;
+// for 2 and 3 the lanes' test becomes the band of that group size.
 template <int K>
-__device__ __forceinline__ bool pair_steps(Lanes<K>& L, uint32_t bmax) {
-    if (bmax == 0) return false;
-    bool small = false;
-#pragma unroll
-    for (int k = 0; k < K; ++k) small |= (L.d[k] >> L.s[k]) <= 2ull * bmax;
-    if (__ballot(small) != 0) return false;
+__device__ __forceinline__ int group_steps(Lanes<K>& L, uint32_t bmax, bool triples) {
+    if (bmax == 0) return 1;
+    bool no2 = false, no3 = !triples || bmax >= (1u << 24);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const uint64_t B = (uint64_t)bmax << L.s[k];
+        const uint64_t p = L.d[k] >> L.s[k];
+        no2 |= p <= 2ull * bmax;
+        no3 |= (p >> 14) <= (uint64_t)bmax * bmax;
+    }
+    if (__ballot(no2) != 0) return 1;
+    const int g = __ballot(no3) != 0 ? 2 : 3;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t B = (g == 3 ? (uint64_t)bmax * bmax : (uint64_t)bmax) << L.s[k];
         L.c[k] = ~B;
         L.thr[k] = L.d[k] - 2 * B - 1;
     }
-    return true;
+    return g;
 }
 
 // r[k] <- r[k] * n mod d[k] up to the rare correction, for all k.
@@ -373,10 +385,11 @@ __device__ __forceinline__ bool step(uint64_t (&r)[K], uint32_t n, const Lanes<K
 // Per-lane re-test and record, out of line: the walk reaches it only through
 // the wavefront-uniform hit branch, so the compiler cannot fold the per-lane
 // condition back into an exec-masked region on the hot path.  r is canonical;
-// +-1 is a hit at step n, and (m != 0) +-n << s a hit at the untested step m.
-__device__ __noinline__ void record_lane(uint64_t r, uint64_t d, uint32_t s, bool live, uint32_t m,
-                                         uint32_t n, int32_t type, pfc_factor* fac, uint32_t fcap,
-                                         pfc_ctrl* ctrl) {
+// +-1 is a hit at step n, (m != 0) +-n << s a hit at the untested step m, and
+// (m2 != 0) +-(m * n) << s a hit at the untested step m2 before m.
+__device__ __noinline__ void record_lane(uint64_t r, uint64_t d, uint32_t s, bool live, uint32_t m2,
+                                         uint32_t m, uint32_t n, int32_t type, pfc_factor* fac,
+                                         uint32_t fcap, pfc_ctrl* ctrl) {
     if (!live) return;
     const uint64_t one = 1ull << s;
     if (r == one || r == d - one) record_factor(fac, fcap, ctrl, d >> s, n, r != one, type);
@@ -384,13 +397,17 @@ __device__ __noinline__ void record_lane(uint64_t r, uint64_t d, uint32_t s, boo
         const uint64_t b = (uint64_t)n << s;
         if (r == b || r == d - b) record_factor(fac, fcap, ctrl, d >> s, m, r != b, type);
     }
+    if (m2) {
+        const uint64_t b = ((uint64_t)m * n) << s;
+        if (r == b || r == d - b) record_factor(fac, fcap, ctrl, d >> s, m2, r != b, type);
+    }
 }
 
-// The slow path of a tested step n (m: the untested step before it, or 0):
-// applies the second correction where it is due, then records the hits.
+// The slow path of a tested step n (m, m2: the untested steps before it, or
+// 0): applies the second correction where it is due, then records the hits.
 template <int K>
-__device__ __forceinline__ void settle(uint64_t (&r)[K], uint32_t m, uint32_t n, int32_t type,
-                                       const Lanes<K>& L, pfc_factor* fac, uint32_t fcap,
+__device__ __forceinline__ void settle(uint64_t (&r)[K], uint32_t m2, uint32_t m, uint32_t n,
+                                       int32_t type, const Lanes<K>& L, pfc_factor* fac, uint32_t fcap,
                                        pfc_ctrl* ctrl) {
     uint64_t hits = 0;
 #pragma unroll
@@ -402,11 +419,15 @@ __device__ __forceinline__ void settle(uint64_t (&r)[K], uint32_t m, uint32_t n,
             const uint64_t b = (uint64_t)n << L.s[k];
             hits |= __ballot(r[k] == b) | __ballot(r[k] == L.d[k] - b);
         }
+        if (m2) {
+            const uint64_t b = ((uint64_t)m * n) << L.s[k];
+            hits |= __ballot(r[k] == b) | __ballot(r[k] == L.d[k] - b);
+        }
     }
     if (hits) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            record_lane(r[k], L.d[k], L.s[k], L.live[k], m, n, type, fac, fcap, ctrl);
+            record_lane(r[k], L.d[k], L.s[k], L.live[k], m2, m, n, type, fac, fcap, ctrl);
     }
 }
 
@@ -423,7 +444,7 @@ struct WalkStream {
 template <int K>
 __device__ __forceinline__ void single(uint64_t (&r)[K], uint32_t n, int32_t type, const Lanes<K>& L,
                                        const WalkStream& W) {
-    if (step<K>(r, n, L)) settle<K>(r, 0, n, type, L, W.fac, W.fcap, W.ctrl);
+    if (step<K>(r, n, L)) settle<K>(r, 0, 0, n, type, L, W.fac, W.fcap, W.ctrl);
 }
 
 // Steps m (untested) and n (tested) of a chain.
@@ -432,12 +453,25 @@ __device__ __forceinline__ void pair(uint64_t (&r)[K], uint32_t m, uint32_t n, i
                                      const Lanes<K>& L, const WalkStream& W) {
     mul_step<K>(r, m, L);
     untested<K>(r);
-    if (step<K>(r, n, L)) settle<K>(r, m, n, type, L, W.fac, W.fcap, W.ctrl);
+    if (step<K>(r, n, L)) settle<K>(r, 0, m, n, type, L, W.fac, W.fcap, W.ctrl);
 }
 
-template <int MODE, int K, bool PAIRED>
+// Steps m2, m (untested) and n (tested) of a chain.
+template <int K>
+__device__ __forceinline__ void triple(uint64_t (&r)[K], uint32_t m2, uint32_t m, uint32_t n,
+                                       int32_t type, const Lanes<K>& L, const WalkStream& W) {
+    mul_step<K>(r, m2, L);
+    untested<K>(r);
+    mul_step<K>(r, m, L);
+    untested<K>(r);
+    if (step<K>(r, n, L)) settle<K>(r, m2, m, n, type, L, W.fac, W.fcap, W.ctrl);
+}
+
+// G: steps per test (group_steps); 3 is used by P mode only.
+template <int MODE, int K, int G>
 __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K], const Lanes<K>& L,
                                            uint32_t j0, uint32_t j1, const WalkStream& W) {
+    constexpr bool PAIRED = G >= 2;
     if (MODE == 0) {
         // Four steps per trip: one scalar compare-and-branch per four updates.
         uint32_t j = j0;
@@ -465,6 +499,24 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
             const uint32_t jal = min(j1, (j0 + 15u) & ~15u);
             for (; j < jal; ++j) single<K>(ra, __builtin_amdgcn_readfirstlane(list[j]), PFC_PRIMORIAL, L, W);
             const ConstU32x16 blocks = (ConstU32x16)W.list;
+            if (G == 3) {
+                // Three blocks (48 primes) per trip: 16 triples.  What is
+                // left takes pairs and single steps below; the triple band
+                // contains the pair and single bands, so they stay exact.
+                for (; j + 48 <= j1; j += 48) {
+                    const U32x16 q0 = blocks[j >> 4], q1 = blocks[(j >> 4) + 1], q2 = blocks[(j >> 4) + 2];
+#pragma unroll
+                    for (int t = 0; t < 48; t += 3) {
+                        uint32_t n[3];
+#pragma unroll
+                        for (int u = 0; u < 3; ++u) {
+                            const int i = t + u;
+                            n[u] = __builtin_amdgcn_readfirstlane(i < 16 ? q0[i] : i < 32 ? q1[i - 16] : q2[i - 32]);
+                        }
+                        triple<K>(ra, n[0], n[1], n[2], PFC_PRIMORIAL, L, W);
+                    }
+                }
+            }
             for (; j + 16 <= j1; j += 16) {
                 const U32x16 q = blocks[j >> 4];
                 if (PAIRED) {
@@ -528,7 +580,7 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
                 if (pend && word) {
                     const uint32_t n = nword + (uint32_t)__builtin_ctz(word);
                     word &= word - 1u;
-                    if (step<K>(rc, n, L)) settle<K>(rc, pend, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
+                    if (step<K>(rc, n, L)) settle<K>(rc, 0, pend, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
                     pend = 0;
                 }
                 while (word & (word - 1u)) {                            // two or more left
@@ -553,16 +605,24 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
             j = jend;
         }
         // A last untested composite step is tested alone.
-        if (PAIRED && pend) settle<K>(rc, 0, pend, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
+        if (PAIRED && pend) settle<K>(rc, 0, 0, pend, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
     }
 }
 
-// Walks steps [j0, j1) with pairs where the wavefront allows them.
+// Walks steps [j0, j1) in groups of as many steps per test as the wavefront
+// allows (triples in P mode only).
 template <int MODE, int K>
 __device__ __forceinline__ void walk_range(uint64_t (&ra)[K], uint64_t (&rb)[K], Lanes<K>& L,
                                            uint32_t j0, uint32_t j1, uint32_t bmax, const WalkStream& W) {
-    if (pair_steps<K>(L, bmax)) walk_steps<MODE, K, true>(ra, rb, L, j0, j1, W);
-    else walk_steps<MODE, K, false>(ra, rb, L, j0, j1, W);
+    const int g = group_steps<K>(L, bmax, MODE == 1);
+    if constexpr (MODE == 1) {
+        if (g == 3) {
+            walk_steps<MODE, K, 3>(ra, rb, L, j0, j1, W);
+            return;
+        }
+    }
+    if (g >= 2) walk_steps<MODE, K, 2>(ra, rb, L, j0, j1, W);
+    else walk_steps<MODE, K, 1>(ra, rb, L, j0, j1, W);
 }
 
 __host__ __device__ constexpr int walk_mode(uint32_t modes) {

@@ -597,3 +597,29 @@ def test_walk_launches_end_at_hits(orc, dev, modes):
     raw = np.frombuffer(fac.download(16 * int(u[1])), dtype=orc.FACTOR_DTYPE)
     got = sorted((int(r["p"]), int(r["nc"]), int(r["type"])) for r in raw if sympy.isprime(int(r["p"])))
     assert got == sorted((int(r["p"]), int(r["nc"]), int(r["type"])) for r in ref.factors)
+
+
+@pytest.mark.gpu
+def test_primorial_triples_find_planted_factors(orc, dev):
+    """P-mode walks over the primes in [101, 400) with every p above
+    2^14 * 397^2 test their steps in triples; p-windows around known factors
+    of n# +- 1 (tests/golden/primorial_factors.json, hits at all three
+    positions of a triple) give the oracle's results, with the planted record
+    among them, run alone and batched."""
+    import json
+    import os
+    import pfcsieve_amd as PK
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "primorial_factors.json")
+    with open(path) as f:
+        vecs = json.load(f)["vectors"]
+    descs = [(P, v["p"] - 3000, v["p"] + 3000, 101, 400) for v in vecs]
+    refs = []
+    for desc in descs:
+        r = orc.run(*desc)
+        refs.append((r.checksum, r.primecount, [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in r.factors]))
+    for desc, v, ref in zip(descs, vecs, refs):
+        got = PK.run_sieve(*desc, device=dev, verify_threads=8)
+        assert (got.checksum, got.primecount, got.factors) == ref, v
+        assert (v["p"], v["c"] * v["n"], 1) in got.factors, v
+    for g, ref in zip(PK.run_batch(descs, device=dev, verify_threads=8), refs):
+        assert (g.checksum, g.primecount, g.factors) == ref

@@ -293,3 +293,30 @@ def test_verify_against_reference_build(orc):
         p, nc = int(f["p"]), int(f["nc"])
         assert orc.verify(p, abs(nc), -1 if nc < 0 else 1, 1)
         assert orc.ref_verify(p, abs(nc), -1 if nc < 0 else 1, 1, 2001)
+
+
+def _primorial_fixtures():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "primorial_factors.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def test_primorial_factor_fixtures(orc):
+    """The planted primorial factors (tests/golden/primorial_factors.json):
+    each p is prime, divides n# + c by big integers, lies above the bound
+    2^14 * 397^2 where the device walk tests its steps in triples, and the
+    oracle reports it in a small p-window around it."""
+    d = _primorial_fixtures()
+    assert (d["nmin"], d["nmax"]) == (101, 400)
+    primes = [int(q) for q in sympy.primerange(101, 400)]
+    assert {v["index"] % 3 for v in d["vectors"]} == {0, 1, 2}
+    for v in d["vectors"]:
+        n, c, p = v["n"], v["c"], v["p"]
+        assert primes[v["index"]] == n
+        assert sympy.isprime(p) and p > 2**14 * 397**2
+        assert (sympy.primorial(n, nth=False) + c) % p == 0
+        r = orc.run(2, p - 200, p + 200, 101, 400)
+        got = [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in r.factors]
+        assert (p, c * n, 1) in got, v
