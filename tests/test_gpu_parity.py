@@ -623,3 +623,29 @@ def test_primorial_triples_find_planted_factors(orc, dev):
         assert (v["p"], v["c"] * v["n"], 1) in got.factors, v
     for g, ref in zip(PK.run_batch(descs, device=dev, verify_threads=8), refs):
         assert (g.checksum, g.primecount, g.factors) == ref
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("modes,lo,width,nmin,nmax,cut", [
+    (F | Cm, 10**12, 2 * 10**8, 101, 10**6, 73_212_347),            # the -! -c benchmark shape
+    (P, 2**62 - 4 * 10**8, 4 * 10**8, 101, 10**7, 151_515_151),      # the primorial shape near 2^62
+])
+def test_full_n_range_additive_over_p(dev, modes, lo, width, nmin, nmax, cut):
+    """At the benchmark configurations' full n-ranges, on p-windows too long
+    for the oracle (1.4e13 and 6.2e12 updates), results add over a p-split
+    that falls inside a segment, with the two parts cut into segments of
+    another size: primecounts and checksums (mod 2^64) of [lo, lo + cut) and
+    [lo + cut, lo + width) sum to the whole window's, and their factor lists
+    concatenate to its list."""
+    import pfcsieve_amd as PK
+    whole = PK.run_sieve(modes, lo, lo + width, nmin, nmax, device=dev, verify_threads=8)
+    dev.set_tuning(seg_candidates=1 << 21)
+    try:
+        a = PK.run_sieve(modes, lo, lo + cut, nmin, nmax, device=dev, verify_threads=8)
+        b = PK.run_sieve(modes, lo + cut, lo + width, nmin, nmax, device=dev, verify_threads=8)
+    finally:
+        dev.set_tuning()
+    assert whole.primecount > 5 * 10**6 and a.segments > 1 and b.segments > 1
+    assert a.primecount + b.primecount == whole.primecount
+    assert (a.checksum + b.checksum) % 2**64 == whole.checksum
+    assert sorted(a.factors + b.factors) == whole.factors
