@@ -136,10 +136,12 @@ def mix_compare(dev, units, verify_threads=16, progress=False):
             "rounds": rounds, "results_identical": sig_plain == sig_batch}
 
 
-def serve_run(dev, units, max_steps=50_000, progress=False):
+def serve_run(dev, units, max_steps=50_000, round_updates=0, progress=False):
     """Served stream: the units arrive as a seeded Poisson process at 80% of
     the batch capacity measured first on the same units; admission by HBM
-    footprint; reports throughput and p50/p99 job latency."""
+    footprint; rounds of at most max_steps steps per segment, or of about
+    round_updates p*n updates when that is set; reports throughput and p50/p99
+    job latency."""
     import pfcsieve_amd as P
     import pfcsieve_amd.serve as S
     from pfcsieve_amd import workload as W
@@ -152,7 +154,7 @@ def serve_run(dev, units, max_steps=50_000, progress=False):
     arrivals = W.poisson_arrivals(len(units), rate, seed=777)
     info = dev.info()
     srv = S.Server(dev, hbm_budget=int(info["hbm_bytes"] * 0.8), max_steps=max_steps, verify_threads=16,
-                   progress=say)
+                   progress=say, round_updates=round_updates)
     for k, (u, a) in enumerate(zip(units, arrivals)):
         srv.submit(S.Request(k, u.desc(), a))
     st = srv.run()
@@ -163,7 +165,8 @@ def serve_run(dev, units, max_steps=50_000, progress=False):
     return {"jobs": st.jobs, "arrival_rate_per_s": rate, "wall_s": st.wall_s, "rounds": st.rounds,
             "updates_per_s": upd / st.wall_s, "p50_latency_s": st.pct(50), "p99_latency_s": st.pct(99),
             "peak_resident_bytes": st.peak_resident_bytes, "results_equal_batch": same,
-            "max_steps_per_round": max_steps}
+            "round_policy": (f"about {round_updates:.3g} updates per round" if round_updates
+                             else f"at most {max_steps} steps per segment per round")}
 
 
 def verify_baseline(dev):
@@ -299,6 +302,9 @@ def main():
                     help="units of the seeded 256-unit batched mix to compare plain vs batched (0 = skip)")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--configs", action="store_true", help="also measure the other single-GPU workloads (whole small job, primorial segment, single modes, set-up)")
+    ap.add_argument("--serve-round-updates", type=float, default=4e11,
+                    help="serve with rounds of about this many p*n updates, ~0.1 s of walk "
+                         "(0: at most 50,000 steps per segment per round)")
     ap.add_argument("--serve-jobs", type=int, default=0,
                     help="jobs of the seeded served stream to run through the serving loop (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -386,7 +392,7 @@ def main():
     if args.serve_jobs:
         from pfcsieve_amd import workload as W
         units = W.mix(10_000, seed=2024)[: args.serve_jobs]
-        line["served"] = serve_run(dev, units, progress=args.progress)
+        line["served"] = serve_run(dev, units, round_updates=int(args.serve_round_updates), progress=args.progress)
         line["served"]["stream"] = ("first %d of 10,000 seeded jobs (splitmix64 seed 2024, same unit distribution "
                                     "as the batched mix), Poisson arrivals at 0.8x measured batch capacity"
                                     % args.serve_jobs)

@@ -168,6 +168,14 @@ int pfc_batcher_pending(const pfc_batcher* b, size_t* n);
  * round.  Jobs that finished in this round are written to done[0..cap) and
  * counted in *ndone, and leave the batcher. */
 int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** done, size_t cap, size_t* ndone);
+
+/* A round sized by work instead of steps: each walking segment takes the
+ * step count that gives about max_updates p*n updates over all of them (at
+ * least 256 steps), so that rounds, and the wait of a job admitted between
+ * them, stay short while large jobs are resident.  No counterpart in the
+ * reference (serving). */
+int pfc_batcher_round_work(pfc_batcher* b, uint64_t max_updates, pfc_job** done, size_t cap,
+                           size_t* ndone);
 int pfc_batcher_destroy(pfc_batcher* b);
 /* Runs jobs to completion through one batcher. */
 int pfc_batch_run(pfc_job** jobs, size_t njobs);

@@ -84,6 +84,7 @@ SIGNATURES = {
     "pfc_batcher_add": (_int, [_vp, _vp]),
     "pfc_batcher_pending": (_int, [_vp, C.POINTER(_sz)]),
     "pfc_batcher_round": (_int, [_vp, _u32, C.POINTER(_vp), _sz, C.POINTER(_sz)]),
+    "pfc_batcher_round_work": (_int, [_vp, _u64, C.POINTER(_vp), _sz, C.POINTER(_sz)]),
     "pfc_batcher_destroy": (_int, [_vp]),
     "pfc_harvest": (_int, [C.POINTER(Factor), _sz, _int, C.POINTER(_u64), C.POINTER(_sz)]),
     "pfc_format_factor": (_int, [C.POINTER(Factor), C.c_char_p, _sz]),

@@ -1235,8 +1235,13 @@ extern "C" int pfc_batcher_pending(const pfc_batcher* b, size_t* n) {
     return PFC_OK;
 }
 
-extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** done, size_t cap,
-                                 size_t* ndone) {
+// Steps per segment of a work-budgeted round, at least this many.
+constexpr uint32_t kMinRoundSteps = 256;
+
+// One round; max_updates != 0 replaces max_steps by the step count that gives
+// about max_updates p*n updates over all walking segments (F|C counted twice).
+static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_updates, pfc_job** done,
+                         size_t cap, size_t* ndone) {
     pfc::g_err[0] = 0;
     if (!b || !ndone) return set_error(PFC_ERR_INVALID, "batcher_round: null pointer");
     *ndone = 0;
@@ -1285,6 +1290,13 @@ extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** d
             s.jcur = s.job->jbegin;
             s.stage = 1;
         }
+    }
+    if (max_updates) {
+        uint64_t cand = 0;
+        for (const pfc_batcher::Slot& s : b->slots)
+            if (s.stage == 1) cand += (uint64_t)s.ncand * (s.job->d_rb ? 2 : 1);
+        const uint64_t steps = cand ? max_updates / cand : 0xFFFFFFFFull;
+        max_steps = (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, std::max<uint64_t>(steps, kMinRoundSteps));
     }
     // 2. work items: one range of items per segment (64*K candidates each),
     // ranges ordered by their items' cost, longest first, so that the queue
@@ -1375,6 +1387,17 @@ extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** d
     }
     b->slots.swap(keep);
     return PFC_OK;
+}
+
+extern "C" int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** done, size_t cap,
+                                 size_t* ndone) {
+    return batcher_round(b, max_steps, 0, done, cap, ndone);
+}
+
+extern "C" int pfc_batcher_round_work(pfc_batcher* b, uint64_t max_updates, pfc_job** done, size_t cap,
+                                      size_t* ndone) {
+    if (max_updates == 0) return set_error(PFC_ERR_INVALID, "batcher_round_work: max_updates is 0");
+    return batcher_round(b, 0xFFFFFFFFu, max_updates, done, cap, ndone);
 }
 
 extern "C" int pfc_batcher_destroy(pfc_batcher* b) {

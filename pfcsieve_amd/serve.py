@@ -6,8 +6,11 @@ admitted jobs' HBM footprints (pfc_job_footprint: prime state + tables +
 factor buffer) stays within the budget; admitted jobs keep all their state on
 the device and share every persistent-kernel launch with the jobs already
 running.  A round walks at most ``max_steps`` multiplier steps per segment,
-so newly admitted work starts within one round.  Finished jobs are harvested
-on the host (verify, isPrime filter, checksum) and their latency recorded.
+or, with ``round_updates``, the step count that gives about that many p*n
+updates over all resident segments (pfc_batcher_round_work), which keeps
+rounds short while large jobs are resident; either way newly admitted work
+starts within one round.  Finished jobs are harvested on the host (verify,
+isPrime filter, checksum) and their latency recorded.
 """
 from __future__ import annotations
 
@@ -47,11 +50,12 @@ class ServeStats:
 
 class Server:
     def __init__(self, dev: Device, hbm_budget: int, max_steps: int = 20000, verify_threads: int = 2,
-                 progress=None):
+                 progress=None, round_updates: int = 0):
         self.progress = progress    # optional callable receiving one line per finished job
         self.dev = dev
         self.budget = hbm_budget
         self.max_steps = max_steps
+        self.round_updates = round_updates
         self.verify_threads = verify_threads
         self.batcher = Batcher(dev)
         self.waiting: List[Request] = []
@@ -94,7 +98,10 @@ class Server:
                 nxt = self.waiting[0].arrival
                 time.sleep(max(0.0, min(0.05, nxt - self._now())))
                 continue
-            finished = self.batcher.round(self.max_steps)
+            if self.round_updates:
+                finished = self.batcher.round_work(self.round_updates)
+            else:
+                finished = self.batcher.round(self.max_steps)
             st.rounds += 1
             now = self._now()
             for job in finished:

@@ -269,6 +269,14 @@ class Batcher:
         check(lib().pfc_batcher_round(self._h, max_steps, arr, cap, C.byref(nd)))
         return [self._jobs.pop(arr[i]) for i in range(min(nd.value, cap))]
 
+    def round_work(self, max_updates: int) -> List[Job]:
+        """One round of about max_updates p*n updates (pfc_batcher_round_work)."""
+        cap = max(len(self._jobs), 1)
+        arr = (C.c_void_p * cap)()
+        nd = C.c_size_t()
+        check(lib().pfc_batcher_round_work(self._h, max_updates, arr, cap, C.byref(nd)))
+        return [self._jobs.pop(arr[i]) for i in range(min(nd.value, cap))]
+
     def destroy(self):
         if self._h:
             lib().pfc_batcher_destroy(self._h)

@@ -18,9 +18,12 @@ REQS = [
 ]
 
 
-def test_served_results_equal_standalone(orc, dev):
+@pytest.mark.parametrize("round_updates", [0, 3 * 10**6])
+def test_served_results_equal_standalone(orc, dev, round_updates):
+    """Rounds of at most 400 steps per segment, or of about 3e6 updates
+    (pfc_batcher_round_work: the step count follows the resident work)."""
     import pfcsieve_amd.serve as S
-    srv = S.Server(dev, hbm_budget=1 << 40, max_steps=400, verify_threads=8)
+    srv = S.Server(dev, hbm_budget=1 << 40, max_steps=400, verify_threads=8, round_updates=round_updates)
     for k, (d, t) in enumerate(REQS):
         srv.submit(S.Request(k, d, t))
     st = srv.run()
