@@ -28,7 +28,7 @@ W_MUL = 19  # INT32 multiply instructions per update in the reference formulatio
 # calibration) and WRITE_SIZE counters of the walk on this workload
 # (profiles/k_walk_fc_1e12_pmc.md): p, v and both residues read, both residues
 # written.
-WALK_FETCH_PER_CAND = 28.63
+WALK_FETCH_PER_CAND = 28.66
 WALK_WRITE_PER_CAND = 16.00
 
 

@@ -256,10 +256,12 @@ int pfc_stage_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, 
  *   modes P:    multiplier n = d_list[j]                  (res_a)
  * Composite flags: bit (n - nbase) of d_composite (u32 words).
  * Records (p, -+n, type) when a residue becomes +-1, up to fcap records.
- * Residues are left canonical at the end of the call.  (Where every p of a
- * wavefront exceeds twice the largest multiplier j1 - 1, F, C and F|C steps
- * are tested in pairs; P steps are tested one by one here, since the call
- * cannot bound d_list's values.  The records are the same either way.) */
+ * The residues must be canonical (below p << clz64(p)), as pfc_stage_setup
+ * and this call leave them.  (Where every p of a wavefront exceeds twice the
+ * largest multiplier j1 - 1, F, C and F|C steps are tested in pairs, and a
+ * pair with both multipliers at most 65,536 is one step by their product;
+ * P steps are tested one by one here, since the call cannot bound d_list's
+ * values.  The records are the same either way.) */
 int pfc_stage_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list,
                    const uint32_t* d_composite, uint32_t nbase, const uint64_t* d_p,
                    const uint32_t* d_v, uint64_t* d_res_a, uint64_t* d_res_b, pfc_ctrl* d_ctrl,

@@ -456,6 +456,20 @@ __device__ __forceinline__ void pair(uint64_t (&r)[K], uint32_t m, uint32_t n, i
     if (step<K>(r, n, L)) settle<K>(r, 0, m, n, type, L, W.fac, W.fcap, W.ctrl);
 }
 
+// Steps m and n of a chain as one step by m * n < 2^32: the product of the
+// input (a tested step's result, so below d) and m * n is below d * 2^32, as
+// the 3-by-2 remainder needs, and the result and its test are those of
+// pair(): a hit at m leaves +-n << s, one at n +-1 << s.
+template <int K>
+__device__ __forceinline__ void pair_product(uint64_t (&r)[K], uint32_t m, uint32_t n, int32_t type,
+                                             const Lanes<K>& L, const WalkStream& W) {
+    if (step<K>(r, m * n, L)) settle<K>(r, 0, m, n, type, L, W.fac, W.fcap, W.ctrl);
+}
+
+// Pairs of multipliers m < n <= kProductLimit have m * n < 2^32
+// (65535 * 65536 = 2^32 - 2^16).
+constexpr uint32_t kProductLimit = 65536;
+
 // Steps m2, m (untested) and n (tested) of a chain.
 template <int K>
 __device__ __forceinline__ void triple(uint64_t (&r)[K], uint32_t m2, uint32_t m, uint32_t n,
@@ -474,7 +488,17 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
     constexpr bool PAIRED = G >= 2;
     if (MODE == 0) {
         // Four steps per trip: one scalar compare-and-branch per four updates.
+        // Trips whose last multiplier is at most kProductLimit take each pair
+        // as one step by its product.
         uint32_t j = j0;
+        if (PAIRED) {
+            const uint32_t jp = min(j1, kProductLimit + 1u);
+            for (; j + 4 <= jp; j += 4) {
+                const uint32_t n0 = __builtin_amdgcn_readfirstlane(j);
+                pair_product<K>(ra, n0, n0 + 1, PFC_FACTORIAL, L, W);
+                pair_product<K>(ra, n0 + 2, n0 + 3, PFC_FACTORIAL, L, W);
+            }
+        }
         for (; j + 4 <= j1; j += 4) {
             uint32_t n[4];
 #pragma unroll
@@ -552,7 +576,9 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
         // word); the composite chain (res_b in F|C, res_a in C) takes the set
         // bits two at a time with a scalar bit scan, one branch per pair, and
         // carries an odd one over to the next word in `pend`.  The chains are
-        // independent, so their order within a word does not matter.
+        // independent, so their order within a word does not matter.  In a
+        // word whose multipliers are all at most kProductLimit each pair is
+        // one step by its product.
         const ConstU32 comp = (ConstU32)W.composite;
         uint64_t (&rc)[K] = (MODE == 2) ? rb : ra;
         uint32_t pend = 0;
@@ -565,8 +591,15 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
             const uint32_t lo = j - nword, hi = jend - nword;          // bits [lo, hi) of the word
             uint32_t word = __builtin_amdgcn_readfirstlane(comp[wbase >> 5]);
             word &= (~0u << lo) & (hi >= 32u ? ~0u : (1u << hi) - 1u);
+            const bool small = nword + 32u <= kProductLimit + 1u;
             if (MODE == 2) {
-                if (PAIRED && lo == 0u && hi == 32u) {
+                if (PAIRED && lo == 0u && hi == 32u && small) {
+#pragma unroll
+                    for (uint32_t t = 0; t < 32; t += 2) {
+                        const uint32_t m = __builtin_amdgcn_readfirstlane(nword + t);
+                        pair_product<K>(ra, m, m + 1, PFC_FACTORIAL, L, W);
+                    }
+                } else if (PAIRED && lo == 0u && hi == 32u) {
 #pragma unroll
                     for (uint32_t t = 0; t < 32; t += 2)
                         pair<K>(ra, __builtin_amdgcn_readfirstlane(nword + t),
@@ -583,12 +616,22 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
                     if (step<K>(rc, n, L)) settle<K>(rc, 0, pend, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
                     pend = 0;
                 }
-                while (word & (word - 1u)) {                            // two or more left
-                    const uint32_t m = nword + (uint32_t)__builtin_ctz(word);
-                    word &= word - 1u;
-                    const uint32_t n = nword + (uint32_t)__builtin_ctz(word);
-                    word &= word - 1u;
-                    pair<K>(rc, m, n, PFC_COMPOSITORIAL, L, W);
+                if (small) {
+                    while (word & (word - 1u)) {                        // two or more left
+                        const uint32_t m = nword + (uint32_t)__builtin_ctz(word);
+                        word &= word - 1u;
+                        const uint32_t n = nword + (uint32_t)__builtin_ctz(word);
+                        word &= word - 1u;
+                        pair_product<K>(rc, m, n, PFC_COMPOSITORIAL, L, W);
+                    }
+                } else {
+                    while (word & (word - 1u)) {                        // two or more left
+                        const uint32_t m = nword + (uint32_t)__builtin_ctz(word);
+                        word &= word - 1u;
+                        const uint32_t n = nword + (uint32_t)__builtin_ctz(word);
+                        word &= word - 1u;
+                        pair<K>(rc, m, n, PFC_COMPOSITORIAL, L, W);
+                    }
                 }
                 if (word) {
                     pend = nword + (uint32_t)__builtin_ctz(word);
