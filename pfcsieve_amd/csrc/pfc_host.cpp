@@ -81,6 +81,9 @@ struct HostTable {
 };
 
 constexpr u128 kChunkLimit = (u128)1 << 63;
+// Widest n-range (nmax - nmin) whose composites are also listed for the walk
+// (4 bytes each); wider C and F|C ranges walk the bitmap alone.
+constexpr uint32_t kMaxCompositeList = 1u << 27;
 
 HostTable build_table(uint32_t kind, uint32_t N) {
     HostTable T;
@@ -469,6 +472,8 @@ struct pfc_job {
     HostTable ta, tb;
     std::vector<uint32_t> list;       // primorial multipliers
     std::vector<uint32_t> composite;  // bitmap over n - nmin (C and F|C)
+    std::vector<uint32_t> clist;      // the composites in [nmin, nmax), increasing (C and F|C)
+    std::vector<uint32_t> cprefix;    // count of clist entries before each bitmap word
     uint32_t jbegin = 0, jend = 0;    // walk step range
     uint64_t steps_per_cand = 0;      // multiplier applications per candidate over the walk
     // device memory
@@ -476,6 +481,7 @@ struct pfc_job {
     uint32_t *d_ga = nullptr, *d_gb = nullptr;
     uint32_t* d_list = nullptr;
     uint32_t* d_comp = nullptr;
+    uint32_t *d_clist = nullptr, *d_cprefix = nullptr;
     uint64_t* d_p = nullptr;
     uint32_t* d_v = nullptr;
     uint64_t *d_one = nullptr, *d_ra = nullptr, *d_rb = nullptr;
@@ -579,6 +585,18 @@ int build_walk_data(pfc_job* j) {
             j->composite[off >> 5] &= ~(1u << (off & 31));
         }
         j->steps_per_cand += w - pr.size();
+        // The walk reads the composites 16 at a time from this list; the
+        // prefix counts let it find a launch's part of the list on the device.
+        if (w <= kMaxCompositeList) {
+            j->clist.reserve(w - pr.size());
+            j->cprefix.assign(j->composite.size(), 0);
+            for (uint32_t k = 0; k < j->composite.size(); ++k) {
+                j->cprefix[k] = (uint32_t)j->clist.size();
+                uint32_t word = j->composite[k];
+                if ((uint64_t)k * 32 + 32 > w) word &= k * 32 >= w ? 0u : (1u << (w - k * 32)) - 1u;
+                for (; word; word &= word - 1) j->clist.push_back(d.nmin + k * 32 + (uint32_t)__builtin_ctz(word));
+            }
+        }
     }
     if (d.modes & PFC_MODE_FACTORIAL) j->steps_per_cand += d.nmax - d.nmin;
     return PFC_OK;
@@ -590,7 +608,7 @@ uint64_t footprint_of(const pfc_job_desc* d, const pfc_tuning& t) {
     uint64_t per = 8 + 4 + 8 + 8 + (two ? 8 : 0);
     uint64_t bytes = (uint64_t)s.cap * per + default_fcap(d) * sizeof(pfc_factor) + sizeof(pfc_ctrl);
     const uint64_t w = (uint64_t)(d->nmax - d->nmin);
-    if (d->modes & PFC_MODE_COMPOSITORIAL) bytes += w / 8 + 8;
+    if (d->modes & PFC_MODE_COMPOSITORIAL) bytes += w / 8 + 8 + (w <= kMaxCompositeList ? 4 * w + w / 8 + 8 : 0);
     if (d->modes & PFC_MODE_PRIMORIAL) bytes += 4 * (uint64_t)(1.3 * (double)w / log((double)d->nmax) + 64);
     // start tables: about one 32-bit chunk per prime below nmin
     bytes += 8 * (uint64_t)(1.3 * (double)d->nmin / log((double)d->nmin) + 64);
@@ -702,6 +720,8 @@ extern "C" int pfc_job_create(pfc_device* dev, const pfc_job_desc* desc, pfc_job
     if (!rc && two) rc = upload(j, &j->d_gb, j->tb.groups, "table B groups");
     if (!rc && !j->list.empty()) rc = upload(j, &j->d_list, j->list, "multiplier list");
     if (!rc && !j->composite.empty()) rc = upload(j, &j->d_comp, j->composite, "composite map");
+    if (!rc && !j->cprefix.empty()) rc = upload(j, &j->d_clist, j->clist, "composite list");
+    if (!rc && !j->cprefix.empty()) rc = upload(j, &j->d_cprefix, j->cprefix, "composite counts");
     if (!rc) {
         e = hipStreamSynchronize(j->stream);
         if (e != hipSuccess) rc = hip_fail(e, "table upload");
@@ -748,8 +768,8 @@ static int build_graph(pfc_job* j) {
     const uint32_t chunk = std::max<uint32_t>(j->tuning.walk_chunk, 1);
     for (uint64_t a = j->jbegin; a < j->jend && !rc; a += chunk) {
         const uint32_t b = (uint32_t)std::min<uint64_t>(j->jend, a + chunk);
-        rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, d.nmin, j->d_p, j->d_v,
-                              j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
+        rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, j->d_clist, j->d_cprefix, d.nmin,
+                              j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
                               cap, j->tuning.primes_per_lane, j->tuning.block_size, walk_bound(j, b), st);
     }
     if (!rc)
@@ -802,8 +822,8 @@ static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
     (void)hipEventRecord(j->evw0, st);
     for (uint64_t a = j->jbegin; a < j->jend; a += chunk, ++launches) {
         const uint32_t b = (uint32_t)std::min<uint64_t>(j->jend, a + chunk);
-        rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, d.nmin, j->d_p, j->d_v,
-                              j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
+        rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, j->d_clist, j->d_cprefix, d.nmin,
+                              j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
                               cap, j->tuning.primes_per_lane, j->tuning.block_size, walk_bound(j, b), st);
         if (rc) return rc;
     }
@@ -1016,8 +1036,8 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
             for (uint32_t blk : blocks) {
                 if (rc) break;
                 (void)hipEventRecord(j->ev0, st);
-                rc = pfc::launch_walk(d.modes, j->jbegin, j->jbegin + steps, j->d_list, j->d_comp, d.nmin,
-                                      j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac,
+                rc = pfc::launch_walk(d.modes, j->jbegin, j->jbegin + steps, j->d_list, j->d_comp, j->d_clist,
+                                      j->d_cprefix, d.nmin, j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac,
                                       (uint32_t)j->fcap, cap, ppl, blk, walk_bound(j, j->jbegin + steps), st);
                 (void)hipEventRecord(j->ev1, st);
                 e = hipEventSynchronize(j->ev1);
@@ -1043,7 +1063,7 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
             (void)hipEventRecord(j->ev0, st);
             for (uint32_t a = 0; a < total && !rc; a += c)
                 rc = pfc::launch_walk(d.modes, j->jbegin + a, j->jbegin + std::min(total, a + c), j->d_list,
-                                      j->d_comp, d.nmin, j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl,
+                                      j->d_comp, j->d_clist, j->d_cprefix, d.nmin, j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl,
                                       j->d_fac, (uint32_t)j->fcap, cap, win.primes_per_lane, win.block_size,
                                       walk_bound(j, j->jbegin + std::min(total, a + c)), st);
             (void)hipEventRecord(j->ev1, st);
@@ -1152,6 +1172,8 @@ pfc_batch_job batch_view(pfc_job* j) {
     b.gb = j->d_gb;
     b.list = j->d_list;
     b.composite = j->d_comp;
+    b.clist = j->d_clist;
+    b.cprefix = j->d_cprefix;
     return b;
 }
 

@@ -26,6 +26,8 @@ struct pfc_batch_job {
     const uint32_t* gb;
     const uint32_t* list;
     const uint32_t* composite;
+    const uint32_t* clist;     // C, F|C (optional): composites in [nmin, nmax), increasing
+    const uint32_t* cprefix;   //   and the count of them before each composite bitmap word
 };
 
 // One work item: 64*K consecutive candidates of a job from index `first`,
@@ -57,9 +59,11 @@ int launch_head_dev(const void* d_bounds, uint64_t max_span, uint64_t* d_p, uint
 int launch_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uint64_t* d_one,
                  uint64_t* d_ra, uint64_t* d_rb, const pfc_setup_table* a,
                  const pfc_setup_table* b, uint32_t cap, hipStream_t st);
+// d_clist / d_cprefix: optional composite list of the n-range and its count
+// before each bitmap word (both or neither); C and F|C walk it when given.
 int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list,
-                const uint32_t* d_comp, uint32_t nbase, const uint64_t* d_p, const uint32_t* d_v,
-                uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
+                const uint32_t* d_comp, const uint32_t* d_clist, const uint32_t* d_cprefix, uint32_t nbase,
+                const uint64_t* d_p, const uint32_t* d_v, uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
                 pfc_factor* d_fac, uint32_t fcap, uint32_t cap, uint32_t ppl, uint32_t block,
                 uint32_t bmax, hipStream_t st);
 int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint32_t* d_v,

@@ -263,6 +263,8 @@ struct WalkArgs {
     uint32_t bmax;                    // largest multiplier of those steps (0: not known)
     const uint32_t* list;             // P, C: multipliers
     const uint32_t* composite;        // F|C: composite bitmap, bit (n - nbase)
+    const uint32_t* clist;            // C, F|C (optional): composite list and its
+    const uint32_t* cprefix;          //   count before each bitmap word
     uint32_t nbase;
     const uint64_t* P;
     const uint32_t* V;
@@ -434,6 +436,8 @@ __device__ __forceinline__ void settle(uint64_t (&r)[K], uint32_t m2, uint32_t m
 struct WalkStream {
     const uint32_t* list;       // P: multipliers
     const uint32_t* composite;  // C, F|C: composite bitmap over n - nbase
+    const uint32_t* clist;      // C, F|C (optional): the composites in increasing order
+    const uint32_t* cprefix;    //   and, per bitmap word, the count of composites before it
     uint32_t nbase;
     pfc_factor* fac;
     uint32_t fcap;
@@ -481,97 +485,138 @@ __device__ __forceinline__ void triple(uint64_t (&r)[K], uint32_t m2, uint32_t m
     if (step<K>(r, n, L)) settle<K>(r, m2, m, n, type, L, W.fac, W.fcap, W.ctrl);
 }
 
+// Factorial steps n = j for j in [j0, j1).  Four steps per trip: one scalar
+// compare-and-branch per four updates.  Trips whose last multiplier is at
+// most kProductLimit take each pair as one step by its product.
+template <int K, int G>
+__device__ __forceinline__ void factorial_steps(uint64_t (&r)[K], const Lanes<K>& L, uint32_t j0,
+                                                uint32_t j1, const WalkStream& W) {
+    constexpr bool PAIRED = G >= 2;
+    uint32_t j = j0;
+    if (PAIRED) {
+        const uint32_t jp = min(j1, kProductLimit + 1u);
+        for (; j + 4 <= jp; j += 4) {
+            const uint32_t n0 = __builtin_amdgcn_readfirstlane(j);
+            pair_product<K>(r, n0, n0 + 1, PFC_FACTORIAL, L, W);
+            pair_product<K>(r, n0 + 2, n0 + 3, PFC_FACTORIAL, L, W);
+        }
+    }
+    for (; j + 4 <= j1; j += 4) {
+        uint32_t n[4];
+#pragma unroll
+        for (uint32_t t = 0; t < 4; ++t) n[t] = __builtin_amdgcn_readfirstlane(j + t);
+        if (PAIRED) {
+            pair<K>(r, n[0], n[1], PFC_FACTORIAL, L, W);
+            pair<K>(r, n[2], n[3], PFC_FACTORIAL, L, W);
+        } else {
+#pragma unroll
+            for (uint32_t t = 0; t < 4; ++t) single<K>(r, n[t], PFC_FACTORIAL, L, W);
+        }
+    }
+    for (; j < j1; ++j) single<K>(r, __builtin_amdgcn_readfirstlane(j), PFC_FACTORIAL, L, W);
+}
+
+// Steps by the listed multipliers list[j], j in [j0, j1), in increasing order
+// (the primes of P, the composites of C).  They are fetched 16 at a time with
+// one 64-byte scalar block load (s_load_dwordx16) when the list is 64-byte
+// aligned, which device allocations are: one address computation and one
+// load per 16 steps keeps the scalar pipe free for the step bookkeeping.  A
+// block whose last (largest) multiplier is at most kProductLimit takes its
+// pairs as product steps.
+template <int K, int G>
+__device__ __forceinline__ void list_steps(uint64_t (&r)[K], const uint32_t* list, uint32_t j0, uint32_t j1,
+                                           int32_t type, const Lanes<K>& L, const WalkStream& W) {
+    constexpr bool PAIRED = G >= 2;
+    const ConstU32 lst = (ConstU32)list;
+    uint32_t j = j0;
+    if ((reinterpret_cast<uintptr_t>(list) & 63u) == 0) {
+        const uint32_t jal = min(j1, (j0 + 15u) & ~15u);
+        for (; j < jal; ++j) single<K>(r, __builtin_amdgcn_readfirstlane(lst[j]), type, L, W);
+        const ConstU32x16 blocks = (ConstU32x16)list;
+        if (G == 3) {
+            // Three blocks (48 multipliers) per trip: 16 triples.  What is
+            // left takes pairs and single steps below; the triple band
+            // contains the pair and single bands, so they stay exact.
+            for (; j + 48 <= j1; j += 48) {
+                const U32x16 q0 = blocks[j >> 4], q1 = blocks[(j >> 4) + 1], q2 = blocks[(j >> 4) + 2];
+#pragma unroll
+                for (int t = 0; t < 48; t += 3) {
+                    uint32_t n[3];
+#pragma unroll
+                    for (int u = 0; u < 3; ++u) {
+                        const int i = t + u;
+                        n[u] = __builtin_amdgcn_readfirstlane(i < 16 ? q0[i] : i < 32 ? q1[i - 16] : q2[i - 32]);
+                    }
+                    triple<K>(r, n[0], n[1], n[2], type, L, W);
+                }
+            }
+        }
+        for (; j + 16 <= j1; j += 16) {
+            const U32x16 q = blocks[j >> 4];
+            if (PAIRED) {
+                if (__builtin_amdgcn_readfirstlane(q[15]) <= kProductLimit) {
+#pragma unroll
+                    for (int t = 0; t < 16; t += 2) {
+                        const uint32_t m = __builtin_amdgcn_readfirstlane(q[t]);
+                        pair_product<K>(r, m, __builtin_amdgcn_readfirstlane(q[t + 1]), type, L, W);
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < 16; t += 2)
+                        pair<K>(r, __builtin_amdgcn_readfirstlane(q[t]), __builtin_amdgcn_readfirstlane(q[t + 1]),
+                                type, L, W);
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 16; ++t) single<K>(r, __builtin_amdgcn_readfirstlane(q[t]), type, L, W);
+            }
+        }
+    } else {
+        for (; j + 16 <= j1; j += 16) {
+            uint32_t q[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) q[t] = __builtin_amdgcn_readfirstlane(lst[j + t]);
+            if (PAIRED) {
+#pragma unroll
+                for (int t = 0; t < 16; t += 2) pair<K>(r, q[t], q[t + 1], type, L, W);
+            } else {
+#pragma unroll
+                for (int t = 0; t < 16; ++t) single<K>(r, q[t], type, L, W);
+            }
+        }
+    }
+    for (; j < j1; ++j) single<K>(r, __builtin_amdgcn_readfirstlane(lst[j]), type, L, W);
+}
+
+// The list index of the first composite at or above n, nbase <= n <= the
+// bitmap's end: the composites before n's bitmap word plus the set bits of
+// that word below n.
+__device__ __forceinline__ uint32_t comp_index(const WalkStream& W, uint32_t n) {
+    const uint32_t off = n - W.nbase;
+    const uint32_t w = off >> 5;
+    const uint32_t below = __builtin_amdgcn_readfirstlane(((ConstU32)W.composite)[w]) & ((1u << (off & 31u)) - 1u);
+    return __builtin_amdgcn_readfirstlane(((ConstU32)W.cprefix)[w]) + (uint32_t)__builtin_popcount(below);
+}
+
 // G: steps per test (group_steps); 3 is used by P mode only.
 template <int MODE, int K, int G>
 __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K], const Lanes<K>& L,
                                            uint32_t j0, uint32_t j1, const WalkStream& W) {
     constexpr bool PAIRED = G >= 2;
     if (MODE == 0) {
-        // Four steps per trip: one scalar compare-and-branch per four updates.
-        // Trips whose last multiplier is at most kProductLimit take each pair
-        // as one step by its product.
-        uint32_t j = j0;
-        if (PAIRED) {
-            const uint32_t jp = min(j1, kProductLimit + 1u);
-            for (; j + 4 <= jp; j += 4) {
-                const uint32_t n0 = __builtin_amdgcn_readfirstlane(j);
-                pair_product<K>(ra, n0, n0 + 1, PFC_FACTORIAL, L, W);
-                pair_product<K>(ra, n0 + 2, n0 + 3, PFC_FACTORIAL, L, W);
-            }
-        }
-        for (; j + 4 <= j1; j += 4) {
-            uint32_t n[4];
-#pragma unroll
-            for (uint32_t t = 0; t < 4; ++t) n[t] = __builtin_amdgcn_readfirstlane(j + t);
-            if (PAIRED) {
-                pair<K>(ra, n[0], n[1], PFC_FACTORIAL, L, W);
-                pair<K>(ra, n[2], n[3], PFC_FACTORIAL, L, W);
-            } else {
-#pragma unroll
-                for (uint32_t t = 0; t < 4; ++t) single<K>(ra, n[t], PFC_FACTORIAL, L, W);
-            }
-        }
-        for (; j < j1; ++j) single<K>(ra, __builtin_amdgcn_readfirstlane(j), PFC_FACTORIAL, L, W);
+        factorial_steps<K, G>(ra, L, j0, j1, W);
     } else if (MODE == 1) {
-        // The primes are fetched 16 at a time with one 64-byte scalar block
-        // load (s_load_dwordx16) when the list is 64-byte aligned, which
-        // device allocations are: one address computation and one load per
-        // 16 steps keeps the scalar pipe free for the step bookkeeping.
-        const ConstU32 list = (ConstU32)W.list;
-        uint32_t j = j0;
-        if ((reinterpret_cast<uintptr_t>(W.list) & 63u) == 0) {
-            const uint32_t jal = min(j1, (j0 + 15u) & ~15u);
-            for (; j < jal; ++j) single<K>(ra, __builtin_amdgcn_readfirstlane(list[j]), PFC_PRIMORIAL, L, W);
-            const ConstU32x16 blocks = (ConstU32x16)W.list;
-            if (G == 3) {
-                // Three blocks (48 primes) per trip: 16 triples.  What is
-                // left takes pairs and single steps below; the triple band
-                // contains the pair and single bands, so they stay exact.
-                for (; j + 48 <= j1; j += 48) {
-                    const U32x16 q0 = blocks[j >> 4], q1 = blocks[(j >> 4) + 1], q2 = blocks[(j >> 4) + 2];
-#pragma unroll
-                    for (int t = 0; t < 48; t += 3) {
-                        uint32_t n[3];
-#pragma unroll
-                        for (int u = 0; u < 3; ++u) {
-                            const int i = t + u;
-                            n[u] = __builtin_amdgcn_readfirstlane(i < 16 ? q0[i] : i < 32 ? q1[i - 16] : q2[i - 32]);
-                        }
-                        triple<K>(ra, n[0], n[1], n[2], PFC_PRIMORIAL, L, W);
-                    }
-                }
-            }
-            for (; j + 16 <= j1; j += 16) {
-                const U32x16 q = blocks[j >> 4];
-                if (PAIRED) {
-#pragma unroll
-                    for (int t = 0; t < 16; t += 2)
-                        pair<K>(ra, __builtin_amdgcn_readfirstlane(q[t]), __builtin_amdgcn_readfirstlane(q[t + 1]),
-                                PFC_PRIMORIAL, L, W);
-                } else {
-#pragma unroll
-                    for (int t = 0; t < 16; ++t)
-                        single<K>(ra, __builtin_amdgcn_readfirstlane(q[t]), PFC_PRIMORIAL, L, W);
-                }
-            }
-        } else {
-            for (; j + 16 <= j1; j += 16) {
-                uint32_t q[16];
-#pragma unroll
-                for (int t = 0; t < 16; ++t) q[t] = __builtin_amdgcn_readfirstlane(list[j + t]);
-                if (PAIRED) {
-#pragma unroll
-                    for (int t = 0; t < 16; t += 2) pair<K>(ra, q[t], q[t + 1], PFC_PRIMORIAL, L, W);
-                } else {
-#pragma unroll
-                    for (int t = 0; t < 16; ++t) single<K>(ra, q[t], PFC_PRIMORIAL, L, W);
-                }
-            }
-        }
-        for (; j < j1; ++j) single<K>(ra, __builtin_amdgcn_readfirstlane(list[j]), PFC_PRIMORIAL, L, W);
+        list_steps<K, G>(ra, W.list, j0, j1, PFC_PRIMORIAL, L, W);
+    } else if (W.clist) {
+        // F|C (MODE 2) and C (MODE 3) with the composite list: the chains
+        // are independent, so F|C walks its factorial chain, then its
+        // composite chain, over the same n-range.
+        if (MODE == 2) factorial_steps<K, G>(ra, L, j0, j1, W);
+        list_steps<K, G>(MODE == 2 ? rb : ra, W.clist, comp_index(W, j0), comp_index(W, j1), PFC_COMPOSITORIAL,
+                         L, W);
     } else {
-        // F|C (MODE 2) and C (MODE 3): composite flags come from the bitmap,
-        // one 32-bit word per 32 steps, limited to [j0, j1).  The factorial
+        // F|C (MODE 2) and C (MODE 3) without the list (the stage entry
+        // point): composite flags come from the bitmap, one 32-bit word per 32 steps, limited to [j0, j1).  The factorial
         // chain of F|C takes every n of the word (16 fixed pairs for a whole
         // word); the composite chain (res_b in F|C, res_a in C) takes the set
         // bits two at a time with a scalar bit scan, one branch per pair, and
@@ -694,7 +739,7 @@ __global__ __launch_bounds__(256) void k_walk(const WalkArgs A) {
         ra[k] = A.RA[src];
         rb[k] = (MODE == 2) ? A.RB[src] : 0;
     }
-    const WalkStream W{A.list, A.composite, A.nbase, A.fac, A.fcap, A.ctrl};
+    const WalkStream W{A.list, A.composite, A.clist, A.cprefix, A.nbase, A.fac, A.fcap, A.ctrl};
     walk_range<MODE, K>(ra, rb, L, A.j0, A.j1, A.bmax, W);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -793,7 +838,7 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
             rb[k] = two ? J.RB[src] : 0;
         }
     }
-    const WalkStream W{J.list, J.composite, J.nmin, J.fac, J.fcap, J.ctrl};
+    const WalkStream W{J.list, J.composite, J.clist, J.cprefix, J.nmin, J.fac, J.fcap, J.ctrl};
     switch (walk_mode(J.modes)) {
         case 0: walk_range<0, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
         case 1: walk_range<1, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
@@ -991,13 +1036,16 @@ static void walk_launch(const WalkArgs& A, uint32_t block, hipStream_t st) {
 }
 
 int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list,
-                const uint32_t* d_comp, uint32_t nbase, const uint64_t* d_p, const uint32_t* d_v,
-                uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
+                const uint32_t* d_comp, const uint32_t* d_clist, const uint32_t* d_cprefix, uint32_t nbase,
+                const uint64_t* d_p, const uint32_t* d_v, uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
                 pfc_factor* d_fac, uint32_t fcap, uint32_t cap, uint32_t ppl, uint32_t block,
                 uint32_t bmax, hipStream_t st) {
     if (j1 <= j0 || cap == 0) return PFC_OK;
+    if ((d_clist == nullptr) != (d_cprefix == nullptr))
+        return set_error(PFC_ERR_INVALID, "walk: composite list without its prefix counts");
     WalkArgs A;
     A.j0 = j0; A.j1 = j1; A.bmax = bmax; A.list = d_list; A.composite = d_comp; A.nbase = nbase;
+    A.clist = d_clist; A.cprefix = d_cprefix;
     A.P = d_p; A.V = d_v; A.RA = d_ra; A.RB = d_rb; A.ctrl = d_ctrl;
     A.fac = d_fac; A.fcap = fcap; A.cap = cap;
     if (block != 64 && block != 128 && block != 256) block = 256;
@@ -1150,7 +1198,7 @@ int pfc_stage_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_l
     // The multiplier of step j is j except in P mode, whose list this call
     // cannot bound without reading it: P steps are then each tested alone.
     const uint32_t bmax = (modes == PFC_MODE_PRIMORIAL || j1 <= j0) ? 0 : j1 - 1;
-    return pfc::launch_walk(modes, j0, j1, d_list, d_composite, nbase, d_p, d_v, d_res_a,
+    return pfc::launch_walk(modes, j0, j1, d_list, d_composite, nullptr, nullptr, nbase, d_p, d_v, d_res_a,
                             d_res_b, d_ctrl, d_fac, fcap, cap, 4, 256, bmax, (hipStream_t)stream);
 }
 
