@@ -25,10 +25,10 @@ sys.path.insert(0, ROOT)
 
 W_MUL = 19  # INT32 multiply instructions per update in the reference formulation
 # HBM bytes per candidate per walk launch, from the FETCH_SIZE (x2.000 copy-kernel
-# calibration) and WRITE_SIZE counters of the walk on this workload
-# (profiles/k_walk_fc_1e12_pmc.md): p, v and both residues read, both residues
-# written.
-WALK_FETCH_PER_CAND = 28.66
+# calibration) and WRITE_SIZE counters of the walk on this workload, 2^24-candidate
+# segments (profiles/k_walk_fc_1e12_pmc.md): p, v and both residues read, both
+# residues written, plus the composite-list reads.
+WALK_FETCH_PER_CAND = 28.90
 WALK_WRITE_PER_CAND = 16.00
 
 
