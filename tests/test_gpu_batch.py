@@ -122,3 +122,18 @@ def test_batch_returns_jobs_with_nothing_left(orc, dev):
     got = PK.run_batch([empty, MIX[2]], device=dev, verify_threads=8)
     cs, pc, fac = _ref(orc, MIX[2])
     assert (got[1].checksum, got[1].primecount, got[1].factors) == (cs, pc, fac)
+
+
+def test_wide_composite_range_walks_bitmap(orc, dev):
+    """A C job whose n-range is wider than 2^27 carries no composite list
+    (only jobs up to that width get one): its walk scans the composite bitmap,
+    alone and in the batch kernel, and both give the oracle's result."""
+    import pfcsieve_amd as PK
+    job = (Cm, 10**12, 10**12 + 600, 101, 101 + (1 << 27) + 4000)
+    ref = orc.run(*job)
+    want = (ref.checksum, ref.primecount, [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in ref.factors])
+    assert ref.primecount > 10
+    plain = PK.run_sieve(*job, device=dev, verify_threads=8)
+    assert (plain.checksum, plain.primecount, plain.factors) == want
+    [bat] = PK.run_batch([job], device=dev, verify_threads=8)
+    assert (bat.checksum, bat.primecount, bat.factors) == want
