@@ -726,6 +726,11 @@ extern "C" int pfc_job_create(pfc_device* dev, const pfc_job_desc* desc, pfc_job
         e = hipStreamSynchronize(j->stream);
         if (e != hipSuccess) rc = hip_fail(e, "table upload");
     }
+    // Only the device copies of the composite data are read from here on (the
+    // list alone can reach 512 MB).
+    std::vector<uint32_t>().swap(j->composite);
+    std::vector<uint32_t>().swap(j->clist);
+    std::vector<uint32_t>().swap(j->cprefix);
     if (!rc && !j->tuning.skip_table_check) rc = queue_table_check(j);
     if (rc) {
         free_job_device(j);
