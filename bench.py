@@ -42,6 +42,21 @@ def composites_in(nmin, nmax):
     return int((~sieve[nmin:nmax]).sum())
 
 
+PRODUCT_LIMIT = 65536  # pairs with both multipliers up to this are one step by their product
+
+
+def mads_per_update(nmin, nmax):
+    """v_mad_u64_u32 instructions the F|C walk of n in [nmin, nmax) issues per
+    update: five per step, and five per pair where a pair is one step by its
+    product (factorial trips of four whose last n is at most PRODUCT_LIMIT;
+    composite-list blocks of 16 whose last composite is).  Assumes the first
+    walk launch starts at nmin and reaches past PRODUCT_LIMIT."""
+    f_prod = max(0, (min(nmax, PRODUCT_LIMIT + 1) - nmin) // 4 * 4)
+    c_prod = composites_in(nmin, min(nmax, PRODUCT_LIMIT + 1)) // 16 * 16
+    steps = (nmax - nmin) + composites_in(nmin, nmax)
+    return 5.0 * (steps - (f_prod + c_prod) / 2) / steps
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -357,6 +372,12 @@ def main():
     walk_launches = s1.walk_launches - s0.walk_launches
     walk_updates = s1.walk_updates - s0.walk_updates
     achieved = walk_updates * W_MUL / (walk_ms * 1e-3) / 1e12
+    # The same launches counted by the multiplies they issue, against the measured
+    # v_mad_u64_u32 lane rate.
+    mpu = mads_per_update(nmin, nmax)
+    issued = {"mad_u64_u32_per_update": mpu, "achieved": walk_updates * mpu / (walk_ms * 1e-3) / 1e12,
+              "peak": rates["mad_u64_u32"] / 1e12, "unit": "Tops/s"}
+    issued["frac"] = issued["achieved"] / issued["peak"]
     line = {
         "metric": "p*n residue updates/sec on 1 MI355X (plain one-range-per-launch form)",
         "value": ups, "unit": "updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -370,6 +391,7 @@ def main():
         "roofline": {"bound": "valu-int32-mul", "achieved": achieved, "peak": p_mul / 1e12,
                      "unit": "Tops/s", "frac": achieved / (p_mul / 1e12),
                      "traffic": (WALK_FETCH_PER_CAND + WALK_WRITE_PER_CAND) * cand / max(args.steps, 1),
+                     "issued": issued,
                      "note": f"walk kernel: {W_MUL} INT32 multiplies per update x updates per launch / "
                              f"average launch duration (HIP events); peak = measured lane-op rate "
                              f"(mad_u64_u32 {rates['mad_u64_u32']:.3e}, mul_lo {rates['mul_lo_u32']:.3e}, "
