@@ -102,6 +102,34 @@ def test_continuous_admission(orc, dev):
             j.destroy()
 
 
+def test_round_work_matches_oracle_and_rejects_zero(orc, dev):
+    """Rounds sized by work (pfc_batcher_round_work: steps = budget / resident
+    candidates, at least 256) give every job the oracle's result; a zero
+    budget is an invalid argument."""
+    import pfcsieve_amd as PK
+    from pfcsieve_amd._native import PfcError
+    b = PK.Batcher(dev)
+    jobs = [PK.Job(dev, *d) for d in MIX]
+    try:
+        with pytest.raises(PfcError, match="max_updates"):
+            b.round_work(0)
+        for j in jobs:
+            b.add(j)
+        done, rounds = [], 0
+        while b.pending():
+            done += b.round_work(2 * 10**7)
+            rounds += 1
+        assert len(done) == len(jobs) and rounds > 2
+        for d, j in zip(MIX, jobs):
+            g = PK.job_result(j, verify_threads=8)
+            cs, pc, fac = _ref(orc, d)
+            assert (g.checksum, g.primecount, g.factors) == (cs, pc, fac), d
+    finally:
+        b.destroy()
+        for j in jobs:
+            j.destroy()
+
+
 def test_batch_returns_jobs_with_nothing_left(orc, dev):
     """A job whose position is already at pmax (an empty p-range, or a job
     that has been run to the end) is handed back by the next round, alone or
