@@ -280,8 +280,9 @@ struct WalkArgs {
 // launch; the multiplier stream is uniform across the wavefront and lives in
 // SGPRs.  Walk modes: 0 = F (n = j), 1 = P (n = list[j]), 2 = F|C (n = j;
 // composite n also multiplies res_b), 3 = C (n = j, composite n only).
-// Composite flags come from a bitmap over n - nbase, one 32-bit word per 32
-// steps.
+// Jobs list their composites (read 16 at a time by scalar block loads, like
+// P's primes); without a list, composite flags come from a bitmap over
+// n - nbase, one 32-bit word per 32 steps.
 //
 // Residues are plain shifted forms x = X << s, so +1 and -1 are one = 2^s and
 // d - one.  A step leaves x == X or x == X + d (the rare second correction of
@@ -616,9 +617,10 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
                          L, W);
     } else {
         // F|C (MODE 2) and C (MODE 3) without the list (the stage entry
-        // point): composite flags come from the bitmap, one 32-bit word per 32 steps, limited to [j0, j1).  The factorial
-        // chain of F|C takes every n of the word (16 fixed pairs for a whole
-        // word); the composite chain (res_b in F|C, res_a in C) takes the set
+        // point, and jobs too wide for one): composite flags come from the
+        // bitmap, one 32-bit word per 32 steps, limited to [j0, j1).  The
+        // factorial chain of F|C takes every n of the word (16 fixed pairs
+        // for a whole word); the composite chain (res_b in F|C, res_a in C) takes the set
         // bits two at a time with a scalar bit scan, one branch per pair, and
         // carries an odd one over to the next word in `pend`.  The chains are
         // independent, so their order within a word does not matter.  In a
