@@ -1,9 +1,10 @@
 """Serving loop: continuous batching of sieve jobs on one MI355X.
 
 Requests arrive over time (``submit``).  Between rounds of the device batcher
-the loop admits waiting requests in arrival order while the sum of the
-admitted jobs' HBM footprints (pfc_job_footprint: prime state + tables +
-factor buffer) stays within the budget; admitted jobs keep all their state on
+the loop admits the requests that have arrived, in arrival order, each one
+whose HBM footprint (pfc_job_footprint: prime state + tables + factor
+buffer) still fits the budget beside the resident jobs (a request that does
+not fit yet waits; smaller ones behind it may go first); admitted jobs keep all their state on
 the device and share every persistent-kernel launch with the jobs already
 running.  A round walks at most ``max_steps`` multiplier steps per segment,
 or, with ``round_updates``, the step count that gives about that many p*n
