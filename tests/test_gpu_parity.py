@@ -546,6 +546,33 @@ def test_factor_buffer_overflow_splits_segments(orc, dev):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", ["plain", "batched"])
+@pytest.mark.parametrize("chunk", [7, 33, 901])
+@pytest.mark.parametrize("modes", [Cm, F | Cm])
+def test_composite_list_short_launches(orc, dev, modes, chunk, form):
+    """Jobs read their composite list 16 entries per block, with single steps
+    before the first whole block and after the last one of each launch (or
+    batch round).  Launches of 7, 33 and 901 n (rounds of as many steps) put
+    those ends all over the list, with pairs and product steps in between
+    (p in [1e5, 3e5) exceeds twice every multiplier below 3001); factor
+    records and checksum against the oracle."""
+    import pfcsieve_amd as PK
+    job = (modes, 10**5, 3 * 10**5, 101, 3001)
+    ref = orc.run(*job)
+    want = (ref.checksum, ref.primecount, [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in ref.factors])
+    assert len(want[2]) > 100
+    if form == "plain":
+        dev.set_tuning(walk_chunk=chunk)
+        try:
+            got = PK.run_sieve(*job, device=dev, verify_threads=8)
+        finally:
+            dev.set_tuning()
+    else:
+        [got] = PK.run_batch([job], device=dev, max_steps=chunk, verify_threads=8)
+    assert (got.checksum, got.primecount, got.factors) == want
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("modes", [F, Cm, F | Cm])
 def test_walk_launches_end_at_hits(orc, dev, modes):
     """The walk tests every second step of a chain when all p exceed twice the
