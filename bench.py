@@ -253,6 +253,19 @@ def other_configs(dev):
                            "steps_per_candidate": steps, "wall_s": dt,
                            "updates_per_s": s.primecount * steps / dt,
                            "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
+    # The same segment with the launch shape and n-chunk length the autotuner
+    # picks for this workload (it leaves them in the device tuning).
+    saved = dev.tuning()
+    best = dev.autotune(2, 2**62 - 4 * 10**9, 2**62, 101, 10**7)
+    j = P.Job(dev, 2, 2**62 - 4 * 10**9, 2**62, 101, 10**7)
+    j.run(1)
+    s = j.stats()
+    j.destroy()
+    dev.set_tuning(**{k: saved[k] for k in ("primes_per_lane", "block_size", "walk_chunk", "seg_candidates",
+                                            "use_graph", "skip_table_check", "lds_segment")})
+    out["primorial_n1e7_p2e62_segment"]["autotuned"] = {
+        "tuning": {k: best[k] for k in ("primes_per_lane", "block_size", "walk_chunk", "lds_segment")},
+        "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
     # Single-mode walks of the uneven mixes at the headline shape: -! and -c
     # alone, n in [101, 1e6), p from 1e12, one p-segment each.
     for name, modes in (("f_segment", 1), ("c_segment", 4)):
