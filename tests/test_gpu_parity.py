@@ -68,6 +68,10 @@ SMALL_JOBS = [
     (P, 10**7, 10**7 + 10**5, 5000, 12000),
     (Cm, 10**7, 10**7 + 10**5, 5000, 12000),
     (F | Cm, 10**7, 10**7 + 10**5, 5000, 9000),
+    # n across 65,536, where pairs stop being product steps (p above twice every n)
+    (F | Cm, 140000, 400000, 65000, 66100),
+    (Cm, 140000, 400000, 65000, 66100),
+    (P, 140000, 400000, 64000, 67000),    # 67000 is composite
 ]
 
 
