@@ -16,6 +16,7 @@ MIX = [
     (P, 101, 20000, 101, 4000),
     (Cm, 10**7, 10**7 + 10**5, 5000, 7000),
     (F, 3, 500, 101, 400),
+    (F | Cm, 140000, 400000, 65000, 66100),   # n across 65,536: product steps, then pairs
 ]
 
 
