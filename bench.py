@@ -342,7 +342,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding over p is "
-                                   "is a design only, not built", "n_gpus": args.gpus}))
+                                   "a design only, not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import pfcsieve_amd as P
@@ -392,11 +392,13 @@ def main():
               "peak": rates["mad_u64_u32"] / 1e12, "unit": "Tops/s"}
     issued["frac"] = issued["achieved"] / issued["peak"]
     line = {
-        "metric": "p*n residue updates/sec on 1 MI355X (plain one-range-per-launch form)",
+        "metric": "p\u00b7n residue updates/sec on 1 MI355X, batched vs plain; % INT32 VALU peak",
         "value": ups, "unit": "updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64 (exact modular integer)", "data": "synthetic (device-generated primes)",
         "config": {"workload": "-! -c, n in [101, 1e6), p from 1e12 (headline config, per p-segment)",
+                   "form": "plain: one range per launch sequence, as the headline configuration is defined "
+                           "(the batched form is under 'batched', the INT32 fraction under 'roofline')",
                    "candidates_timed": cand, "steps_per_candidate": steps_per_cand,
                    "seg_candidates": args.seg_candidates, "tuning": dev.tuning(), "device_ms": dev_ms,
                    "walk_launches": walk_launches, "walk_ms_per_launch": walk_ms / max(walk_launches, 1),
