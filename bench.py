@@ -1,21 +1,42 @@
 """Benchmark of the sieve hot path on one MI355X.
 
-Workload (default): plain factorial + compositorial sieve, -! -c,
-n in [101, 1,000,000), p from 1e12 (the 2e9-wide p-range of the headline
-configuration, processed one p-segment at a time).  One "step" is one
-p-segment of about 2^24 candidates run through generate -> setup -> walk ->
-check, with its factor records drained; inputs are generated on the device.
+Prints ONE JSON line.  The metric is p*n residue updates per second (updates
+= candidates x multiplier steps; F and C walks of a -! -c job counted
+separately), batched against plain, plus the walk's fraction of the INT32
+VALU multiply peak.
 
-Prints ONE JSON line.  `value` is p*n residue updates per second over the
-timed steps (updates = candidates x multiplier steps, F and C walks counted
-separately).  `roofline` prices the updates at 19 INT32 multiplies each (the
-reference's 64-bit Montgomery product, 11 32x32->64 partial products) against
-the measured INT32 multiply lane-op rate of the device.
-`cpu_baseline` times the CPU oracle (OpenMP, all host cores) on a bounded
-p-sample of the same workload.
+Workloads (synthetic: every candidate prime is generated on the device):
+
+* ``value`` -- the batched uneven mix: all 256 units of the seeded mix
+  (splitmix64 seed 12345; modes F/P/C with equal odds, n-width log-U[1e3,1e6]
+  from 101+U[0,1e6], p-width log-U[1e6,1e9] from log-U[1e10,2^63]), every
+  unit one p-segment.  Batched: all units in one batcher, timed from job
+  creation to job destruction; a step is one batcher round (one persistent
+  k_batch launch), and the K rounds are sized by work (round r walks about
+  the remaining updates / (K - r); the last walks everything left), so every
+  step does work and the K steps cover the mix.  Plain (``mix.plain``): the
+  same units one job after another, one range per launch sequence.
+  Warmup: W batcher rounds over four small units of another seed.
+* ``headline`` -- the plain -! -c job over exactly p in [1e12, 1e12+2e9),
+  n in [101, 1e6): one p-segment holding all its ~72.38M candidates, walked
+  in K launches of ceil(999,899 / K) multiplier steps (a step is one walk
+  launch; generate and set-up run before the first, the checksum fold after
+  the last); the whole job is timed.  Warmup: the same n-range on
+  p in [1e12 - 1e8, 1e12), walked in W launches.
+* ``roofline`` -- the walk kernel (k_walk) of the headline job: multiply
+  instructions it issues (v_mad_u64_u32, counted from its instruction
+  sequence per launch) per second of launch time (HIP events on the job's
+  stream) against 256 CUs x 64 lanes x 2.4 GHz.
+* ``served`` -- a prefix of the seeded served stream (splitmix64 seed 2024,
+  units drawn as in the mix), Poisson arrivals at 0.8x the capacity that the
+  batched mix's measured work rate gives for those jobs.
+* ``cpu_baseline`` -- the CPU oracle (OpenMP over candidates) on a bounded
+  p-sample of the headline job, plus the reference's CPU-runnable case in
+  full.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -23,38 +44,252 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-W_MUL = 19  # INT32 multiply instructions per update in the reference formulation
-# HBM bytes per candidate per walk launch, from the FETCH_SIZE (x2.000 copy-kernel
-# calibration) and WRITE_SIZE counters of the walk on this workload, 2^24-candidate
-# segments (profiles/k_walk_fc_1e12_pmc.md): p, v and both residues read, both
-# residues written, plus the composite-list reads.
+PEAK_LANE_OPS = 256 * 64 * 2.4e9   # MI355X: 256 CUs x 64 lanes x 2.4 GHz peak engine clock
+W_MUL_REF = 19                     # INT32 multiplies per update in the reference's 64-bit Montgomery product
+PRODUCT_LIMIT = 65536              # pairs with both multipliers up to this are one step by their product
+# HBM bytes per candidate per walk launch of the headline job (FETCH_SIZE x
+# the copy-kernel calibration, and WRITE_SIZE, of k_walk: p, v and both
+# residues read, both residues written, plus the launch's composite-list
+# reads amortised over its candidates).
 WALK_FETCH_PER_CAND = 28.90
 WALK_WRITE_PER_CAND = 16.00
 
+HEAD_MODES, HEAD_PMIN, HEAD_PMAX, HEAD_NMIN, HEAD_NMAX = 5, 10**12, 10**12 + 2 * 10**9, 101, 10**6
 
-def composites_in(nmin, nmax):
+
+def _progress(on, msg):
+    if on:
+        print(msg, file=sys.stderr, flush=True)
+
+
+def prime_flags(n):
     import numpy as np
-    sieve = np.ones(nmax, dtype=bool)
-    sieve[:2] = False
-    for i in range(2, int(nmax ** 0.5) + 1):
-        if sieve[i]:
-            sieve[i * i::i] = False
-    return int((~sieve[nmin:nmax]).sum())
+    s = np.ones(n + 1, dtype=bool)
+    s[:2] = False
+    for i in range(2, int(n ** 0.5) + 1):
+        if s[i]:
+            s[i * i::i] = False
+    return s
 
 
-PRODUCT_LIMIT = 65536  # pairs with both multipliers up to this are one step by their product
+def walk_mads(nmin, nmax, chunk):
+    """v_mad_u64_u32 instructions per candidate that the F|C walk of
+    n in [nmin, nmax) issues in launches of `chunk` steps, with every step
+    paired (all p above twice the largest n): five per multiplier step, five
+    per pair taken as one step by its product (factorial trips of four whose
+    last n is at most PRODUCT_LIMIT; composite-list blocks of 16, aligned to
+    the list, whose last composite is).  Returns (mads, updates, product
+    updates) per candidate."""
+    import numpy as np
+    isp = prime_flags(nmax)
+    comp = np.nonzero(~isp[nmin:nmax])[0] + nmin          # composites in [nmin, nmax), the C list
+    prod = 0
+    for a in range(nmin, nmax, chunk):
+        b = min(nmax, a + chunk)
+        jp = min(b, PRODUCT_LIMIT + 1)
+        if jp > a:
+            prod += (jp - a) // 4 * 4
+        c0 = int(np.searchsorted(comp, a))
+        c1 = int(np.searchsorted(comp, b))
+        j = min(c1, (c0 + 15) & ~15)
+        while j + 16 <= c1:
+            if comp[j + 15] <= PRODUCT_LIMIT:
+                prod += 16
+            j += 16
+    upd = (nmax - nmin) + len(comp)
+    return 5.0 * upd - 2.5 * prod, upd, prod
 
 
-def mads_per_update(nmin, nmax):
-    """v_mad_u64_u32 instructions the F|C walk of n in [nmin, nmax) issues per
-    update: five per step, and five per pair where a pair is one step by its
-    product (factorial trips of four whose last n is at most PRODUCT_LIMIT;
-    composite-list blocks of 16 whose last composite is).  Assumes the first
-    walk launch starts at nmin and reaches past PRODUCT_LIMIT."""
-    f_prod = max(0, (min(nmax, PRODUCT_LIMIT + 1) - nmin) // 4 * 4)
-    c_prod = composites_in(nmin, min(nmax, PRODUCT_LIMIT + 1)) // 16 * 16
-    steps = (nmax - nmin) + composites_in(nmin, nmax)
-    return 5.0 * (steps - (f_prod + c_prod) / 2) / steps
+def measure_headline(dev, K, W, progress):
+    """The plain -! -c job over exactly [1e12, 1e12 + 2e9): one p-segment,
+    K walk launches; warmup on [1e12 - 1e8, 1e12) in W launches."""
+    import pfcsieve_amd as P
+    nsteps = HEAD_NMAX - HEAD_NMIN
+    chunk = -(-nsteps // K)
+    try:
+        if W:
+            dev.set_tuning(seg_candidates=1 << 27, walk_chunk=-(-nsteps // W))
+            wj = P.Job(dev, HEAD_MODES, HEAD_PMIN - 10**8, HEAD_PMIN, HEAD_NMIN, HEAD_NMAX)
+            wj.run()
+            ws = wj.stats()
+            wj.destroy()
+            _progress(progress, f"headline warmup: {ws.walk_launches} launches over {ws.primecount} candidates")
+        dev.set_tuning(seg_candidates=1 << 27, walk_chunk=chunk)
+        job = P.Job(dev, HEAD_MODES, HEAD_PMIN, HEAD_PMAX, HEAD_NMIN, HEAD_NMAX)
+        t0 = time.perf_counter()
+        job.run()
+        wall = time.perf_counter() - t0
+        s = job.stats()
+        job.destroy()
+    finally:
+        dev.set_tuning()
+    assert s.segments == 1 and s.walk_launches == K, (s.segments, s.walk_launches)
+    _progress(progress, f"headline: {s.primecount} candidates, {wall:.1f} s")
+    return {"workload": "-! -c n in [101, 1e6) p in [1e12, 1e12+2e9): plain, one range per launch sequence",
+            "pmin": HEAD_PMIN, "pmax": HEAD_PMAX, "nmin": HEAD_NMIN, "nmax": HEAD_NMAX,
+            "steps": K, "warmup": W,
+            "step": f"one k_walk launch of {chunk} multiplier steps over every candidate (generate and set-up "
+                    f"before the first, the checksum fold after the last)",
+            "candidates_timed": s.primecount, "steps_per_candidate": s.walk_updates // max(s.primecount, 1),
+            "segments": s.segments, "updates": s.walk_updates, "wall_s": wall, "ms_per_step": wall * 1e3 / K,
+            "updates_per_s": s.walk_updates / wall, "device_ms": s.device_ms,
+            "walk_launches": s.walk_launches, "walk_ms": s.walk_ms,
+            "walk_ms_per_launch": s.walk_ms / s.walk_launches,
+            "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3), "walk_chunk": chunk}
+
+
+def roofline(head, mul_rates):
+    """Roofline figures of k_walk on the headline job."""
+    mads, upd, prod = walk_mads(HEAD_NMIN, HEAD_NMAX, head["walk_chunk"])
+    launches = head["walk_launches"]
+    avg_s = head["walk_ms"] * 1e-3 / launches
+    cand = head["candidates_timed"]
+    achieved = mads * cand / launches / avg_s
+    ups = head["walk_updates_per_s"]
+    # VALU issue slots per update from the instruction sequence: a step is 8
+    # single-rate instructions (5 multiply-adds, the 64-bit add, the quotient
+    # compare, the exec-masked correction add) and 2 double-rate register
+    # moves; a pair of steps adds the band test (an add and a compare); a
+    # product step covers two updates with one step and its test.
+    f_prod = prod / upd
+    slots = (1 - f_prod) * (8 + 1 + 1) + f_prod * (8 + 1 + 2) / 2
+    return {
+        "bound": "valu-int32", "kernel": "k_walk on the headline job",
+        "achieved": achieved / 1e12, "peak": PEAK_LANE_OPS / 1e12, "unit": "Tops/s",
+        "frac": achieved / PEAK_LANE_OPS,
+        "traffic": (WALK_FETCH_PER_CAND + WALK_WRITE_PER_CAND) * cand,
+        "mads_per_update": mads / upd, "updates_per_launch": upd * cand / launches,
+        "avg_launch_ms": avg_s * 1e3,
+        "valu_issue": {"slots_per_update": slots, "achieved": ups * slots / 1e12,
+                       "frac": ups * slots / PEAK_LANE_OPS,
+                       "note": "single-rate instruction slots per update of the walk's instruction sequence "
+                               "(double-rate moves at half a slot) x updates/s, against the same peak"},
+        "reference_equivalent": {"mul_per_update": W_MUL_REF, "achieved": ups * W_MUL_REF / 1e12,
+                                 "frac": ups * W_MUL_REF / PEAK_LANE_OPS,
+                                 "note": "updates/s priced at the reference's 19 INT32 multiplies per update "
+                                         "(its 64-bit Montgomery product); not instructions this walk issues"},
+        "measured_lane_rates": {k: v / 1e12 for k, v in mul_rates.items()},
+        "note": "achieved = v_mad_u64_u32 issued per launch (5 per multiplier step, 5 per product step of "
+                "two multipliers) / average k_walk launch duration (HIP events on the job's stream); peak = "
+                "256 CU x 64 lanes x 2.4 GHz; traffic = HBM bytes per launch (rocprof FETCH_SIZE x copy "
+                "calibration + WRITE_SIZE, per candidate, x candidates per launch)",
+    }
+
+
+def _unit_updates(u):
+    """Walk updates of a unit from its p-width (prime density 1/ln p)."""
+    from pfcsieve_amd import workload as Wl
+    return (u.pmax - u.pmin) / math.log(u.pmin) * Wl.walk_steps(u)
+
+
+def batched_rounds(dev, units, K, progress, tag="batched"):
+    """Runs units through one batcher in K rounds sized by work: round r
+    walks about the estimated remaining updates / (K - r) (pfc_batcher_round_work),
+    the last one everything left (every unit is one p-segment, so that ends
+    the batch).  Returns (wall seconds, rounds, per-job (checksum, primecount,
+    factors), per-job stats)."""
+    import pfcsieve_amd as P
+    t = time.perf_counter()
+    jobs = [P.Job(dev, *u.desc()) for u in units]
+    b = P.Batcher(dev)
+    for j in jobs:
+        b.add(j)
+    left_est = sum(_unit_updates(u) for u in units)
+    rounds = 0
+    while b.pending():
+        if K - rounds <= 1:
+            b.round(0)
+        else:
+            budget = left_est / (K - rounds)
+            b.round_work(max(1, int(budget)))
+            left_est -= budget
+        rounds += 1
+        _progress(progress, f"{tag}: round {rounds} done, {b.pending()} jobs pending, {time.perf_counter() - t:.1f} s")
+    stats = [j.stats() for j in jobs]
+    sigs = [j.result() for j in jobs]
+    for j in jobs:
+        j.destroy()
+    b.destroy()
+    return time.perf_counter() - t, rounds, sigs, stats
+
+
+def measure_mix(dev, K, W, progress, nunits=256):
+    """All units of the seeded 256-unit mix, plain (one job after another,
+    from job creation to destruction) and batched (one batcher, K rounds);
+    every unit's checksum, candidate count and raw factor records must agree."""
+    import pfcsieve_amd as P
+    from pfcsieve_amd import workload as Wl
+    units = Wl.mix(256, seed=12345)[:nunits]
+    steps = [Wl.walk_steps(u) for u in units]
+    dev.set_tuning(seg_candidates=1 << 26)        # every unit (p-width <= 1e9) is one p-segment
+    try:
+        if W:
+            warm = sorted(Wl.mix(64, seed=4242), key=_unit_updates)[:4]
+            batched_rounds(dev, warm, W, False, "warmup")
+        t = time.perf_counter()
+        plain = []
+        for k, u in enumerate(units):
+            j = P.Job(dev, *u.desc())
+            j.run()
+            plain.append(j.result())
+            j.destroy()
+            _progress(progress, f"plain: unit {k + 1}/{len(units)} done, {time.perf_counter() - t:.1f} s")
+        t_plain = time.perf_counter() - t
+        t_batch, rounds, batched, stats = batched_rounds(dev, units, K, progress)
+    finally:
+        dev.set_tuning()
+    cand = [r[1] for r in batched]
+    upd = sum(c * s for c, s in zip(cand, steps))
+    products = sum(c * setup_products(u) for c, u in zip(cand, units))
+    return {"units": len(units), "candidates": sum(cand), "walk_updates": upd, "setup_products": products,
+            "segments": sum(s.segments for s in stats), "plain_s": t_plain, "batched_s": t_batch,
+            "rounds": rounds, "plain_updates_per_s": upd / t_plain, "batched_updates_per_s": upd / t_batch,
+            "batched_over_plain": t_plain / t_batch, "results_identical": plain == batched}
+
+
+_TABLE_COST = {}
+
+
+def setup_products(u):
+    """Modular products per candidate for the start residue X(nmin - 1)."""
+    import pfcsieve_amd as P
+    key = ({1: 0, 2: 1, 4: 2}[u.modes], u.nmin - 1)
+    if key not in _TABLE_COST:
+        _TABLE_COST[key] = P.setup_table_cost(*key)
+    return _TABLE_COST[key]
+
+
+def serve_prefix(dev, njobs, mix, round_updates, progress):
+    """The first njobs of the seeded 10,000-job stream through the serving
+    loop.  Arrival rate: 0.8 x the capacity the batched mix's measured work
+    rate gives for these jobs (work = walk updates + set-up products x the
+    mix's product/update time ratio), instead of a separate capacity run."""
+    import pfcsieve_amd.serve as S
+    from pfcsieve_amd import workload as Wl
+    units = Wl.mix(10_000, seed=2024)[:njobs]
+    # the mix's device time split: walk at the headline walk rate, the rest
+    # set-up; one work unit = one walk update
+    upd_s = mix["batched_s"]
+    ratio = 2.25                                       # set-up product / walk update time, measured on MI355X
+    mix_work = mix["walk_updates"] + ratio * mix["setup_products"]
+    rate = mix_work / upd_s
+    work = sum(_unit_updates(u) + ratio * setup_products(u) * (u.pmax - u.pmin) / math.log(u.pmin)
+               for u in units)
+    lam = 0.8 * njobs / (work / rate)
+    arrivals = Wl.poisson_arrivals(njobs, lam, seed=777)
+    info = dev.info()
+    say = (lambda m: _progress(True, m)) if progress else None
+    srv = S.Server(dev, hbm_budget=int(info["hbm_bytes"] * 0.8), verify_threads=16, progress=say,
+                   round_updates=round_updates)
+    for k, (u, a) in enumerate(zip(units, arrivals)):
+        srv.submit(S.Request(k, u.desc(), a))
+    st = srv.run()
+    upd = sum(r.result.primecount * Wl.walk_steps(units[r.rid]) for r in srv.done)
+    return {"jobs": st.jobs, "stream": f"first {njobs} of 10,000 seeded jobs (splitmix64 seed 2024, units as in "
+                                       f"the mix), Poisson arrivals (seed 777)",
+            "arrival_rate_per_s": lam, "wall_s": st.wall_s, "rounds": st.rounds, "updates_per_s": upd / st.wall_s,
+            "p50_latency_s": st.pct(50), "p99_latency_s": st.pct(99), "peak_resident_bytes": st.peak_resident_bytes,
+            "round_policy": f"about {round_updates:.3g} p*n updates per round"}
 
 
 def _cpu_model():
@@ -68,375 +303,89 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(modes, p0, width, nmin, nmax, steps_per_cand, width3=600_000):
+def cpu_baseline(width):
     """The oracle (C, OpenMP over candidates, all host threads) on a bounded
-    p-sample of the headline configuration, plus the reference's CPU-runnable
-    case in full and a p-sample of the primorial configuration."""
+    p-sample of the headline job, and the reference's CPU-runnable case
+    (-! n [101, 5000) p [1e9, 1.001e9)) in full."""
     import oracle as O
-    from pfcsieve_amd import workload as W
     O.build()
+    steps = walk_mads(HEAD_NMIN, HEAD_NMAX, HEAD_NMAX)[1]
     t = time.perf_counter()
-    r = O.run(modes, p0, p0 + width, nmin, nmax)
+    r = O.run(HEAD_MODES, HEAD_PMIN, HEAD_PMIN + width, HEAD_NMIN, HEAD_NMAX)
     dt = time.perf_counter() - t
-    upd = r.primecount * steps_per_cand
-    out = {"value": upd / dt, "unit": "updates/s", "cores": O.num_threads(), "kind": "port",
-           "sample": f"-! -c p in [{p0}, {p0 + width}) ({r.primecount} candidates), n in [{nmin}, {nmax}); "
-                     f"{dt:.1f} s",
+    out = {"value": r.primecount * steps / dt, "unit": "updates/s", "cores": O.num_threads(), "kind": "port",
+           "sample": f"-! -c p in [{HEAD_PMIN}, {HEAD_PMIN + width}) ({r.primecount} candidates), "
+                     f"n in [{HEAD_NMIN}, {HEAD_NMAX}); {dt:.1f} s",
+           "algorithm": "the oracle's walk: one 64-bit Montgomery product per update (the reference's "
+                        "formulation, iterate.cl:156-198), where the GPU walk divides by 32-bit multipliers "
+                        "(3-by-2 division, paired tests); a GPU/CPU ratio mixes that change with the hardware",
            "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
-    # the reference's CPU-runnable case in full: -! -n 101 -N 5000 -p 1e9 -P 1.001e9
     t = time.perf_counter()
     r1 = O.run(1, 10**9, 10**9 + 10**6, 101, 5000)
     dt1 = time.perf_counter() - t
-    out["factorial_n5000_p1e9_full"] = {"candidates": r1.primecount, "s": dt1, "updates_per_s": r1.primecount * 4899 / dt1,
-                        "checksum": "%016X" % r1.checksum}
-    # primorial p-sample: -# -n 101 -N 1e7 from p = 2^62 - 4e9
-    p3 = 2**62 - 4 * 10**9
-    steps3 = W.walk_steps(W.Unit(2, 0, 0, 101, 10**7))
-    t = time.perf_counter()
-    r3 = O.run(2, p3, p3 + width3, 101, 10**7)
-    dt3 = time.perf_counter() - t
-    out["primorial_n1e7_p2e62_sample"] = {"sample": f"-# p in [{p3}, {p3 + width3}), n in [101, 1e7)",
-                          "candidates": r3.primecount, "s": dt3,
-                          "updates_per_s": r3.primecount * steps3 / dt3}
+    out["factorial_n5000_p1e9_full"] = {"candidates": r1.primecount, "s": dt1,
+                                        "updates_per_s": r1.primecount * 4899 / dt1,
+                                        "checksum": "%016X" % r1.checksum}
     return out
 
 
-def _progress(on, msg):
-    if on:
-        print(msg, file=sys.stderr, flush=True)
-
-
-def mix_compare(dev, units, verify_threads=16, progress=False):
-    """Plain (one job at a time, one range per launch sequence) against batched
-    (all units through one batcher, shared persistent launches) on the same
-    seeded unit mix.  Returns updates/s of both and the ratio."""
+def small_job(dev):
+    """The reference's CPU-runnable case on the GPU, whole job including
+    table self-check and harvest: -! n [101, 5000) p [1e9, 1.001e9)."""
     import pfcsieve_amd as P
-    from pfcsieve_amd import workload as W
-    steps = [W.walk_steps(u) for u in units]
-    # plain
-    t = time.perf_counter()
-    cand_plain, sig_plain = [], []
-    for u in units:
-        j = P.Job(dev, *u.desc())
-        j.run()
-        st = j.stats()
-        cand_plain.append(st.primecount)
-        sig_plain.append((st.dev_checksum, st.primecount, st.raw_factors))
-        j.destroy()
-        _progress(progress, f"plain: unit {len(cand_plain)}/{len(units)} done, {time.perf_counter() - t:.1f} s")
-    t_plain = time.perf_counter() - t
-    # batched (job creation -- host tables, device state -- inside the timed
-    # region, as it is for the plain form)
-    t = time.perf_counter()
-    jobs = [P.Job(dev, *u.desc()) for u in units]
-    b = P.Batcher(dev)
-    for j in jobs:
-        b.add(j)
-    rounds = 0
-    while b.pending():
-        b.round(0)
-        rounds += 1
-        _progress(progress, f"batched: round {rounds} done, {b.pending()} jobs pending, {time.perf_counter() - t:.1f} s")
-    cand_batch = [j.stats().primecount for j in jobs]
-    sig_batch = [(j.stats().dev_checksum, j.stats().primecount, j.stats().raw_factors) for j in jobs]
-    for j in jobs:
-        j.destroy()
-    b.destroy()
-    t_batch = time.perf_counter() - t
-    assert cand_plain == cand_batch
-    upd = sum(c * s for c, s in zip(cand_plain, steps))
-    return {"units": len(units), "candidates": sum(cand_plain), "walk_updates": upd,
-            "plain_s": t_plain, "batched_s": t_batch, "plain_updates_per_s": upd / t_plain,
-            "batched_updates_per_s": upd / t_batch, "batched_over_plain": t_plain / t_batch,
-            "rounds": rounds, "results_identical": sig_plain == sig_batch}
-
-
-def serve_run(dev, units, max_steps=50_000, round_updates=0, progress=False):
-    """Served stream: the units arrive as a seeded Poisson process at 80% of
-    the batch capacity measured first on the same units; admission by HBM
-    footprint; rounds of at most max_steps steps per segment, or of about
-    round_updates p*n updates when that is set; reports throughput and p50/p99
-    job latency."""
-    import pfcsieve_amd as P
-    import pfcsieve_amd.serve as S
-    from pfcsieve_amd import workload as W
-    steps = [W.walk_steps(u) for u in units]
-    t = time.perf_counter()
-    say = (lambda m: _progress(True, m)) if progress else None
-    res = P.run_batch([u.desc() for u in units], device=dev, verify_threads=16, progress=say)
-    cap_s = time.perf_counter() - t
-    rate = 0.8 * len(units) / cap_s
-    arrivals = W.poisson_arrivals(len(units), rate, seed=777)
-    info = dev.info()
-    srv = S.Server(dev, hbm_budget=int(info["hbm_bytes"] * 0.8), max_steps=max_steps, verify_threads=16,
-                   progress=say, round_updates=round_updates)
-    for k, (u, a) in enumerate(zip(units, arrivals)):
-        srv.submit(S.Request(k, u.desc(), a))
-    st = srv.run()
-    by_id = {r.rid: r for r in srv.done}
-    same = all((by_id[k].result.checksum, by_id[k].result.primecount) == (res[k].checksum, res[k].primecount)
-               for k in range(len(units)))
-    upd = sum(r.primecount * s for r, s in zip(res, steps))
-    return {"jobs": st.jobs, "arrival_rate_per_s": rate, "wall_s": st.wall_s, "rounds": st.rounds,
-            "updates_per_s": upd / st.wall_s, "p50_latency_s": st.pct(50), "p99_latency_s": st.pct(99),
-            "peak_resident_bytes": st.peak_resident_bytes, "results_equal_batch": same,
-            "round_policy": (f"about {round_updates:.3g} updates per round" if round_updates
-                             else f"at most {max_steps} steps per segment per round")}
-
-
-def verify_baseline(dev):
-    """The CPU factor verification (getResults' OpenMP verify, cl_sieve.cpp:509-540)
-    on the factor records of self-test case 3 (-! p in [101, 1e5), n in [101, 1e6),
-    42,821 factors), at 2 threads (the -v default) and at all host threads."""
-    import ctypes as C
-    import pfcsieve_amd as P
-    from pfcsieve_amd._native import check, lib
-    j = P.Job(dev, 1, 101, 100000, 101, 1000000)
-    j.run()
-    arr, n = j.raw_factors()
-    j.destroy()
-    out = {"factors": n, "sample": "self-test case 3 factor records"}
-    for threads in (2, os.cpu_count() or 2):
-        cp = (type(arr[0]) * n)(*arr[:n])
-        cs, kept = C.c_uint64(0), C.c_size_t()
-        t = time.perf_counter()
-        check(lib().pfc_harvest(cp, n, threads, C.byref(cs), C.byref(kept)))
-        out[f"threads_{threads}_s"] = time.perf_counter() - t
-    # The reference's own verify() (verifyprime.cpp:149, compiled from its
-    # source into oracle/_ref when the reference tree was present at build
-    # time) on the first records, one call per record on all host threads as
-    # getResults' OpenMP loop does, beside the library's harvest of the same
-    # records (which also sorts, filters by isPrime and sums the checksum).
-    import oracle as orc
-    if orc.ref_available():
-        from concurrent.futures import ThreadPoolExecutor
-        m = min(n, 4000)
-        threads = os.cpu_count() or 2
-        orc.ref_is_prime(3)                     # loads the library before the threads start
-        recs = [(int(f.p), abs(int(f.nc)), 1 if f.nc > 0 else -1, int(f.type)) for f in arr[:m]]
-        t = time.perf_counter()
-        with ThreadPoolExecutor(threads) as ex:
-            ok = list(ex.map(lambda r: orc.ref_verify(r[0], r[1], r[2], r[3], 1000000), recs))
-        t_ref = time.perf_counter() - t
-        cp = (type(arr[0]) * m)(*arr[:m])
-        cs, kept = C.c_uint64(0), C.c_size_t()
-        t = time.perf_counter()
-        check(lib().pfc_harvest(cp, m, threads, C.byref(cs), C.byref(kept)))
-        out["reference_verify"] = {"records": m, "threads": threads, "all_verified": all(ok),
-                                   "reference_s": t_ref, "library_harvest_s": time.perf_counter() - t}
-    return out
-
-
-def other_configs(dev):
-    """Throughput on the other single-GPU workloads: the reference's CPU-runnable
-    case (-! n [101,5000) p [1e9,1.001e9), whole job), the primorial walk near
-    2^62 (-# n [101,1e7), one p-segment), single-mode walks and large-nmin set-up."""
-    import pfcsieve_amd as P
-    out = {}
-    # -! -n 101 -N 5000 -p 1e9 -P 1.001e9 (48,155 candidates), whole job
     t = time.perf_counter()
     r = P.run_sieve(1, 10**9, 10**9 + 10**6, 101, 5000, device=dev, verify_threads=16)
     dt = time.perf_counter() - t
-    out["factorial_n5000_p1e9"] = {"workload": "-! n [101,5000) p [1e9,1.001e9)", "candidates": r.primecount,
-                   "updates": r.primecount * 4899, "wall_s": dt, "updates_per_s": r.primecount * 4899 / dt,
-                   "factors": r.factorcount, "checksum": "%016X" % r.checksum}
-    # -# -n 101 -N 1e7 -p 2^62-4e9 -P 2^62, one segment
-    from pfcsieve_amd import workload as W
-    steps = W.walk_steps(W.Unit(2, 0, 0, 101, 10**7))
-    j = P.Job(dev, 2, 2**62 - 4 * 10**9, 2**62, 101, 10**7)
-    t = time.perf_counter()
-    j.run(1)
-    dt = time.perf_counter() - t
-    s = j.stats()
-    j.destroy()
-    out["primorial_n1e7_p2e62_segment"] = {"workload": "-# n [101,1e7) p from 2^62-4e9, one p-segment", "candidates": s.primecount,
-                           "steps_per_candidate": steps, "wall_s": dt,
-                           "updates_per_s": s.primecount * steps / dt,
-                           "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
-    # The same segment with the launch shape and n-chunk length the autotuner
-    # picks for this workload (it leaves them in the device tuning).
-    saved = dev.tuning()
-    best = dev.autotune(2, 2**62 - 4 * 10**9, 2**62, 101, 10**7)
-    j = P.Job(dev, 2, 2**62 - 4 * 10**9, 2**62, 101, 10**7)
-    j.run(1)
-    s = j.stats()
-    j.destroy()
-    dev.set_tuning(**{k: saved[k] for k in ("primes_per_lane", "block_size", "walk_chunk", "seg_candidates",
-                                            "use_graph", "skip_table_check", "lds_segment")})
-    out["primorial_n1e7_p2e62_segment"]["autotuned"] = {
-        "tuning": {k: best[k] for k in ("primes_per_lane", "block_size", "walk_chunk", "lds_segment")},
-        "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
-    # Single-mode walks of the uneven mixes at the headline shape: -! and -c
-    # alone, n in [101, 1e6), p from 1e12, one p-segment each.
-    for name, modes in (("f_segment", 1), ("c_segment", 4)):
-        steps = W.walk_steps(W.Unit(modes, 0, 0, 101, 10**6))
-        j = P.Job(dev, modes, 10**12, 10**12 + 2 * 10**9, 101, 10**6)
-        j.run(1)
-        s = j.stats()
-        j.destroy()
-        out[name] = {"workload": ("-!" if modes == 1 else "-c") + " n [101,1e6) p from 1e12, one p-segment",
-                     "candidates": s.primecount, "steps_per_candidate": steps,
-                     "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
-    # Start-residue set-up at large nmin (the dominant cost of the uneven mixes):
-    # -! n [1e6, 1e6+256) from p = 1e15, one p-segment.  Set-up time is the
-    # segment's device time minus its walk launches (generate and check are
-    # ~3 ms of it).
-    nchunks, nbits = P.setup_table_size(0, 10**6 - 1)
-    products = nchunks + nbits - 1               # chunk products and squarings
-    j = P.Job(dev, 1, 10**15, 10**15 + 2 * 10**9, 10**6, 10**6 + 256)
-    j.run(1)
-    s = j.stats()
-    j.destroy()
-    setup_s = (s.device_ms - s.walk_ms) * 1e-3
-    out["setup_nmin_1e6"] = {"workload": "-! n [1e6,1e6+256) p from 1e15, one p-segment: start residues "
-                                         "999999! mod p per candidate",
-                             "candidates": s.primecount, "modular_products_per_candidate": products,
-                             "setup_s": setup_s, "candidates_per_s": s.primecount / setup_s,
-                             "products_per_s": s.primecount * products / setup_s}
-    return out
-
-
-def shape_sweep(dev, modes, pmin, nmin, nmax):
-    """Walk rate of one 2^21-candidate headline segment at every launch shape
-    the autotuner considers (primes per lane x workgroup size)."""
-    import pfcsieve_amd as P
-    out = []
-    for ppl in (1, 2, 4):
-        for blk in (64, 128, 256):
-            dev.set_tuning(primes_per_lane=ppl, block_size=blk, seg_candidates=1 << 21)
-            j = P.Job(dev, modes, pmin, pmin + 10**9, nmin, nmax)
-            j.run(1)
-            s = j.stats()
-            j.destroy()
-            out.append({"primes_per_lane": ppl, "block_size": blk,
-                        "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)})
-            _progress(True, f"sweep ppl={ppl} block={blk}: {out[-1]['walk_updates_per_s']:.4e} updates/s")
-    dev.set_tuning()
-    return out
+    return {"workload": "-! n [101,5000) p [1e9,1.001e9)", "candidates": r.primecount, "wall_s": dt,
+            "updates_per_s": r.primecount * 4899 / dt, "factors": r.factorcount, "checksum": "%016X" % r.checksum}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=4, help="K: batcher rounds of the mix, walk launches of the headline")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--seg-candidates", type=int, default=1 << 24)
-    ap.add_argument("--ppl", type=int, default=0)
-    ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--walk-chunk", type=int, default=0, help="multiplier steps per walk launch (0 = default)")
+    ap.add_argument("--mix-units", type=int, default=256, help="units of the seeded mix (256 = the whole mix)")
+    ap.add_argument("--serve-jobs", type=int, default=200, help="jobs of the seeded served stream (0 = skip)")
+    ap.add_argument("--serve-round-updates", type=float, default=4e11)
+    ap.add_argument("--no-headline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-width", type=int, default=1_000_000)
-    ap.add_argument("--mix-units", type=int, default=24,
-                    help="units of the seeded 256-unit batched mix to compare plain vs batched (0 = skip)")
-    ap.add_argument("--autotune", action="store_true")
-    ap.add_argument("--configs", action="store_true", help="also measure the other single-GPU workloads (whole small job, primorial segment, single modes, set-up)")
-    ap.add_argument("--serve-round-updates", type=float, default=4e11,
-                    help="serve with rounds of about this many p*n updates, ~0.1 s of walk "
-                         "(0: at most 50,000 steps per segment per round)")
-    ap.add_argument("--serve-jobs", type=int, default=0,
-                    help="jobs of the seeded served stream to run through the serving loop (0 = skip)")
-    ap.add_argument("--copy-probe", type=int, default=0,
-                    help="run one plain copy of this many bytes first (HBM counter calibration)")
-    ap.add_argument("--progress", action="store_true", help="progress lines on stderr during long mixes")
-    ap.add_argument("--sweep", action="store_true", help="walk rate at every (primes per lane, block) shape")
+    ap.add_argument("--progress", action="store_true", help="progress lines on stderr")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding over p is "
-                                   "a design only, not built", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding over p is a design "
+                                   "only, not built", "n_gpus": args.gpus}))
         sys.exit(2)
+    K, W = max(1, args.steps), max(0, args.warmup)
 
     import pfcsieve_amd as P
-    modes, nmin, nmax = P.MODE_FACTORIAL | P.MODE_COMPOSITORIAL, 101, 1_000_000
-    pmin = 10**12
-    ncomp = composites_in(nmin, nmax)
-    steps_per_cand = (nmax - nmin) + ncomp
-
     dev = P.Device(0)
-    if args.copy_probe:
-        dev.probe_copy(args.copy_probe)
     rates = dev.probe_mul_rates()
-    p_mul = max(rates.values())
-    dev.set_tuning(seg_candidates=args.seg_candidates, primes_per_lane=args.ppl, block_size=args.block,
-                   walk_chunk=args.walk_chunk)
-    if args.autotune:
-        dev.autotune(modes, pmin, pmin + 2 * 10**9, nmin, nmax)
-    total = args.steps + args.warmup
-    # p-range: the headline 2e9 window, or wider if more segments are asked for
-    job = P.Job(dev, modes, pmin, pmin + 2 * 10**9 * max(1, (total + 16) // 17), nmin, nmax)
-    for _ in range(args.warmup):
-        job.run(1)
-    s0 = job.stats()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        job.run(1)
-    t1 = time.perf_counter()
-    s1 = job.stats()
-    cand = s1.primecount - s0.primecount
-    updates = cand * steps_per_cand
-    wall = t1 - t0
-    dev_ms = s1.device_ms - s0.device_ms
-    ups = updates / wall
-    job.destroy()
-
-    # Dominant kernel: the walk.  Its launches are bracketed by HIP events on
-    # the job's stream inside the library; achieved = algorithmic multiplies
-    # of those launches / their summed duration.
-    walk_ms = s1.walk_ms - s0.walk_ms
-    walk_launches = s1.walk_launches - s0.walk_launches
-    walk_updates = s1.walk_updates - s0.walk_updates
-    achieved = walk_updates * W_MUL / (walk_ms * 1e-3) / 1e12
-    # The same launches counted by the multiplies they issue, against the measured
-    # v_mad_u64_u32 lane rate.
-    mpu = mads_per_update(nmin, nmax)
-    issued = {"mad_u64_u32_per_update": mpu, "achieved": walk_updates * mpu / (walk_ms * 1e-3) / 1e12,
-              "peak": rates["mad_u64_u32"] / 1e12, "unit": "Tops/s"}
-    issued["frac"] = issued["achieved"] / issued["peak"]
+    head = None if args.no_headline else measure_headline(dev, K, W, args.progress)
+    mix = measure_mix(dev, K, W, args.progress, args.mix_units) if args.mix_units else None
     line = {
-        "metric": "p\u00b7n residue updates/sec on 1 MI355X, batched vs plain; % INT32 VALU peak",
-        "value": ups, "unit": "updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u64 (exact modular integer)", "data": "synthetic (device-generated primes)",
-        "config": {"workload": "-! -c, n in [101, 1e6), p from 1e12 (headline config, per p-segment)",
-                   "form": "plain: one range per launch sequence, as the headline configuration is defined "
-                           "(the batched form is under 'batched', the INT32 fraction under 'roofline')",
-                   "candidates_timed": cand, "steps_per_candidate": steps_per_cand,
-                   "seg_candidates": args.seg_candidates, "tuning": dev.tuning(), "device_ms": dev_ms,
-                   "walk_launches": walk_launches, "walk_ms_per_launch": walk_ms / max(walk_launches, 1),
-                   "walk_updates_per_s": walk_updates / (walk_ms * 1e-3)},
-        "roofline": {"bound": "valu-int32-mul", "achieved": achieved, "peak": p_mul / 1e12,
-                     "unit": "Tops/s", "frac": achieved / (p_mul / 1e12),
-                     "traffic": (WALK_FETCH_PER_CAND + WALK_WRITE_PER_CAND) * cand / max(args.steps, 1),
-                     "issued": issued,
-                     "note": f"walk kernel: {W_MUL} INT32 multiplies per update x updates per launch / "
-                             f"average launch duration (HIP events); peak = measured lane-op rate "
-                             f"(mad_u64_u32 {rates['mad_u64_u32']:.3e}, mul_lo {rates['mul_lo_u32']:.3e}, "
-                             f"mul_hi {rates['mul_hi_u32']:.3e}); traffic = HBM bytes per walk launch "
-                             f"(PMC bytes per candidate x candidates per segment)"},
+        "metric": "p·n residue updates/sec on 1 MI355X, batched vs plain; % INT32 VALU peak",
+        "value": mix["batched_updates_per_s"] if mix else head["updates_per_s"], "unit": "updates/s",
+        "n_gpus": 1, "steps": mix["rounds"] if mix else K, "warmup": W,
+        "ms_per_step": (mix["batched_s"] / mix["rounds"] if mix else head["wall_s"] / K) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 (exact modular integer)", "data": "synthetic (seeded job mix; device-generated primes)",
+        "config": {"workload": (f"batched uneven mix: {mix['units']} seeded units (splitmix64 seed 12345)"
+                                if mix else "headline -! -c job"),
+                   "form": "batched: all units through one batcher, persistent k_batch launches; a step is one "
+                           "batcher round, the K rounds sized by work; plain beside it under 'mix'",
+                   "mix": mix, "headline": head, "tuning": dev.tuning()},
     }
-    if args.configs:
-        line["other_configs"] = other_configs(dev)
-    if args.sweep:
-        line["shape_sweep"] = shape_sweep(dev, modes, pmin, nmin, nmax)
-        dev.set_tuning(seg_candidates=args.seg_candidates, primes_per_lane=args.ppl, block_size=args.block,
-                       walk_chunk=args.walk_chunk)
-    if args.mix_units:
-        from pfcsieve_amd import workload as W
-        units = W.mix(256, seed=12345)[: args.mix_units]
-        line["batched"] = mix_compare(dev, units, progress=args.progress)
-        line["batched"]["mix"] = ("first %d of 256 seeded units (splitmix64 seed 12345): modes F/P/C equal odds, "
-                                  "n-width log-U[1e3,1e6] from 101+U[0,1e6], p-width log-U[1e6,1e9] from "
-                                  "log-U[1e10,2^63]" % args.mix_units)
-    if args.serve_jobs:
-        from pfcsieve_amd import workload as W
-        units = W.mix(10_000, seed=2024)[: args.serve_jobs]
-        line["served"] = serve_run(dev, units, round_updates=int(args.serve_round_updates), progress=args.progress)
-        line["served"]["stream"] = ("first %d of 10,000 seeded jobs (splitmix64 seed 2024, same unit distribution "
-                                    "as the batched mix), Poisson arrivals at 0.8x measured batch capacity"
-                                    % args.serve_jobs)
+    if head:
+        line["roofline"] = roofline(head, rates)
+    if mix and args.serve_jobs:
+        line["served"] = serve_prefix(dev, args.serve_jobs, mix, int(args.serve_round_updates), args.progress)
+    line["config"]["small_job"] = small_job(dev)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(modes, pmin, args.cpu_width, nmin, nmax, steps_per_cand)
-        line["cpu_baseline"]["gpu_over_cpu"] = ups / line["cpu_baseline"]["value"]
-        line["cpu_baseline"]["verify"] = verify_baseline(dev)
+        line["cpu_baseline"] = cpu_baseline(args.cpu_width)
+        if head:
+            line["cpu_baseline"]["gpu_over_cpu"] = head["updates_per_s"] / line["cpu_baseline"]["value"]
     dev.close()
     print(json.dumps(line), flush=True)
 

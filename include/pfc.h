@@ -31,7 +31,8 @@ extern "C" {
 #define PFC_ERR_HIP (-2)             /* HIP runtime failure (simpleCL.c:347 analogue) */
 #define PFC_ERR_SIEVE (-3)           /* primorial nmax is prime: "CPU sieve failure" (cl_sieve.cpp:1061-1066) */
 #define PFC_ERR_FACTOR_OVERFLOW (-4) /* factor records exceeded capacity (cl_sieve.cpp:484-487) */
-#define PFC_ERR_VALIDATION (-5)      /* device self-check failed (check.cl:62-73, cl_sieve.cpp:476-480) */
+#define PFC_ERR_VALIDATION (-5)      /* device self-check failed: walk incomplete (check.cl:62-73,
+                                        cl_sieve.cpp:476-480), a stored reciprocal, or a start table */
 #define PFC_ERR_VERIFY (-6)          /* CPU factor verification failed (cl_sieve.cpp:509-529) */
 #define PFC_ERR_NOMEM (-7)           /* host or device allocation failed */
 #define PFC_ERR_IO (-8)              /* factors.txt / checkpoint file I/O */
@@ -148,6 +149,12 @@ int pfc_job_sync(pfc_job* job);
 int pfc_job_result(const pfc_job* job, uint64_t* checksum_partial, uint64_t* primecount, pfc_factor* out,
                    size_t cap, size_t* n);
 int pfc_job_destroy(pfc_job* job);
+/* Test hook of the walk-completion check: from now on the job's walk leaves
+ * out the last `steps` multiplier steps of every segment (each segment's
+ * last walk launch, plain, graph or batched, ends that many steps early), so
+ * the drain must fail with PFC_ERR_VALIDATION.  0 restores the full walk.  No
+ * counterpart in the reference; for tests of the guard only. */
+int pfc_job_debug_shorten_walk(pfc_job* job, uint32_t steps);
 
 /* ------------------------------------------------------------------ batching */
 /* No counterpart in the reference, which runs one p-range per launch
@@ -205,15 +212,28 @@ int pfc_parse_uint64(const char* s, uint64_t lo, uint64_t hi, uint64_t* out);
  * and by callers that build their own pipelines.  All pointers are device
  * pointers unless stated.
  *
- * Control block, 32 bytes, zeroed by the caller before a segment:
- *   u32 ncand, nfactor, flags, pad; u64 checksum, primecount. */
+ * Control block, PFC_CTRL_BYTES (48) bytes, zeroed by the caller before a
+ * segment:  u32 ncand, nfactor, flags, pad; u64 checksum, primecount, walked,
+ * reserved.
+ *
+ * `walked` is the walk-completion evidence that replaces the reference's
+ * per-lane multiplier counter (.s7 advanced by one every step and converted
+ * back and compared with nmax-1 at the check, check.cl:62-73; a mismatch is
+ * "gpu validation failure", cl_sieve.cpp:476-480): every wavefront counts the
+ * multiplier applications it performed, in a scalar register inside the step
+ * loops, and adds count x its live candidates once.  After a segment's walk
+ * over steps [j0, j1) it must equal ncand x (multipliers in [j0, j1)); the job
+ * and batch drains compare the two and fail with PFC_ERR_VALIDATION. */
+#define PFC_CTRL_BYTES 48
 typedef struct pfc_ctrl {
     uint32_t ncand;      /* candidates appended (g_primecount[0]) */
     uint32_t nfactor;    /* factor records appended (g_primecount[2]); may exceed capacity */
-    uint32_t flags;      /* bit0 candidate overflow, bit1 validation failure */
+    uint32_t flags;      /* bit0 candidate overflow, bit1 stored reciprocal failed its check */
     uint32_t pad;
     uint64_t checksum;   /* sum of check terms (wrapping) */
     uint64_t primecount; /* sum of ncand over checked segments */
+    uint64_t walked;     /* multiplier applications summed over candidates (walk launches) */
+    uint64_t reserved;
 } pfc_ctrl;
 
 /* Device memory for stage-level callers that have no allocator of their own
@@ -226,7 +246,7 @@ int pfc_memcpy(pfc_device* dev, void* dst, const void* src, size_t bytes, int ki
 int pfc_device_synchronize(pfc_device* dev);
 
 /* (a) Candidates in [lo, hi): primes below 114 plus wheel-30 numbers with no
- * factor 7..113 that pass the base-2 strong PRP (getsegprimes.cl:464-526,
+ * factor 7..113 that pass the base-2 strong PRP (getsegprimes.cl:170-232,
  * addsmallprimes.cl:21-40).  Appends to d_p[0..cap) in no particular order;
  * ctrl->ncand counts all, flag bit0 when more than cap. */
 int pfc_stage_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pfc_ctrl* d_ctrl,
@@ -255,7 +275,11 @@ int pfc_stage_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, 
  *   modes F|C:  n = j for res_a; composite j also res_b   (res_b)
  *   modes P:    multiplier n = d_list[j]                  (res_a)
  * Composite flags: bit (n - nbase) of d_composite (u32 words).
- * Records (p, -+n, type) when a residue becomes +-1, up to fcap records.
+ * Records (p, -+n, type) when a residue becomes +-1, up to fcap records, and
+ * adds the multiplier applications performed (candidates x steps) to
+ * ctrl->walked.  Ranges: F, C and F|C need 2 <= j0 and j1 <= 2^31 - 1 (the
+ * multiplier is j); P needs j1 <= 2^31 - 1 and every d_list value below 2^31;
+ * anything else is PFC_ERR_INVALID.
  * The residues must be canonical (below p << clz64(p)), as pfc_stage_setup
  * and this call leave them.  (Where every p of a wavefront exceeds twice the
  * largest multiplier j1 - 1, F, C and F|C steps are tested in pairs, and a

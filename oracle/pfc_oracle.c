@@ -11,7 +11,7 @@
  *                                          verifyprime.cpp:23-59
  *   base-2 strong PRP .................... kernels/getsegprimes.cl:79-108
  *   candidate set (wheel-30, sieve 7..113, PRP; primes < 114)
- *                                          kernels/getsegprimes.cl:464-526,
+ *                                          kernels/getsegprimes.cl:170-232,
  *                                          kernels/addsmallprimes.cl:21-40,
  *                                          cl_sieve.cpp:1598-1615
  *   wheel offset ......................... cl_sieve.cpp:369-430
@@ -76,7 +76,7 @@ uint64_t orc_madd(uint64_t a, uint64_t b, uint64_t p) {
 
 typedef struct { uint64_t p, q, one, two, nmo, r2; } orc_mont;
 
-/* Per-prime constants as getsegprimes.cl:509-512 and setup.cl:68-73 build them. */
+/* Per-prime constants as getsegprimes.cl:214-217 and setup.cl:68-73 build them. */
 static orc_mont mont_init(uint64_t p) {
     orc_mont m;
     m.p = p;
@@ -216,7 +216,7 @@ static int survives_wheel_sieve(uint64_t c) {
 /* The reference's candidate set for [pmin, pmax): primes below 114 from
  * addsmallprimes (addsmallprimes.cl:25-31, host clamp cl_sieve.cpp:1599-1605),
  * then every wheel-30 number in [max(pmin,114), pmax) with no factor 7..113
- * that passes the base-2 strong PRP (getsegprimes.cl:488-516).  Returns the
+ * that passes the base-2 strong PRP (getsegprimes.cl:193-223).  Returns the
  * count, writes at most cap values in ascending order. */
 size_t orc_candidates(uint64_t pmin, uint64_t pmax, uint64_t* out, size_t cap) {
     size_t n = 0;
@@ -416,7 +416,7 @@ static uint64_t mont_pow_lr(uint64_t base, uint32_t e, uint32_t bit, const orc_m
 }
 
 /* The 8-word record of one candidate after setup, walk and check: the
- * reference's ulong8 (getsegprimes.cl:515; iterate.cl:69,168). */
+ * reference's ulong8 (getsegprimes.cl:221; iterate.cl:69,168). */
 typedef struct { uint64_t s[8]; } orc_rec;
 
 typedef struct { orc_factor* f; size_t n, cap; } fvec;

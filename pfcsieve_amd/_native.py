@@ -51,7 +51,12 @@ class JobStats(C.Structure):
 
 class Ctrl(C.Structure):
     _fields_ = [("ncand", C.c_uint32), ("nfactor", C.c_uint32), ("flags", C.c_uint32),
-                ("pad", C.c_uint32), ("checksum", C.c_uint64), ("primecount", C.c_uint64)]
+                ("pad", C.c_uint32), ("checksum", C.c_uint64), ("primecount", C.c_uint64),
+                ("walked", C.c_uint64), ("reserved", C.c_uint64)]
+
+
+CTRL_BYTES = 48  # PFC_CTRL_BYTES
+assert C.sizeof(Ctrl) == CTRL_BYTES
 
 
 class SetupTable(C.Structure):
@@ -79,6 +84,7 @@ SIGNATURES = {
     "pfc_job_sync": (_int, [_vp]),
     "pfc_job_result": (_int, [_vp, C.POINTER(_u64), C.POINTER(_u64), _vp, _sz, C.POINTER(_sz)]),
     "pfc_job_destroy": (_int, [_vp]),
+    "pfc_job_debug_shorten_walk": (_int, [_vp, _u32]),
     "pfc_batch_run": (_int, [C.POINTER(_vp), _sz]),
     "pfc_batcher_create": (_int, [_vp, C.POINTER(_vp)]),
     "pfc_batcher_add": (_int, [_vp, _vp]),

@@ -336,7 +336,7 @@ PFC_HD uint64_t to_mont(uint64_t x, uint64_t mone, const Modulus& m) {
 // ---------------------------------------------------------------- wheel + small-prime sieve
 
 // The sieve primes of the reference's candidate generator (kernels/
-// getsegprimes.cl:488-491): every prime from 7 to 113.
+// getsegprimes.cl:116-168, 194-197): every prime from 7 to 113.
 constexpr uint32_t kSievePrimes[27] = {7,  11, 13, 17, 19, 23, 29, 31,  37,  41,  43,  47,  53, 59,
                                        61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113};
 

@@ -476,6 +476,7 @@ struct pfc_job {
     std::vector<uint32_t> cprefix;    // count of clist entries before each bitmap word
     uint32_t jbegin = 0, jend = 0;    // walk step range
     uint64_t steps_per_cand = 0;      // multiplier applications per candidate over the walk
+    uint32_t debug_short = 0;         // test hook: steps left out of each segment's last walk launch
     // device memory
     uint64_t *d_ca = nullptr, *d_cb = nullptr;
     uint32_t *d_ga = nullptr, *d_gb = nullptr;
@@ -726,17 +727,19 @@ extern "C" int pfc_job_create(pfc_device* dev, const pfc_job_desc* desc, pfc_job
         e = hipStreamSynchronize(j->stream);
         if (e != hipSuccess) rc = hip_fail(e, "table upload");
     }
-    // Only the device copies of the composite data are read from here on (the
-    // list alone can reach 512 MB).
-    std::vector<uint32_t>().swap(j->composite);
-    std::vector<uint32_t>().swap(j->clist);
-    std::vector<uint32_t>().swap(j->cprefix);
     if (!rc && !j->tuning.skip_table_check) rc = queue_table_check(j);
     if (rc) {
+        // free_job_device waits for the stream first: copies queued from the
+        // host vectors may still be reading them.
         free_job_device(j);
         delete j;
         return rc;
     }
+    // Only the device copies of the composite data are read from here on (the
+    // list alone can reach 512 MB); the stream has drained the uploads.
+    std::vector<uint32_t>().swap(j->composite);
+    std::vector<uint32_t>().swap(j->clist);
+    std::vector<uint32_t>().swap(j->cprefix);
     *out = j;
     return PFC_OK;
 }
@@ -744,11 +747,33 @@ extern "C" int pfc_job_create(pfc_device* dev, const pfc_job_desc* desc, pfc_job
 // Drains the control block and factor records of a finished segment.
 static int drain_segment(pfc_job* j, bool* overflow, uint64_t launches, bool timed);
 
+// Message of a stored reciprocal failing its check at the fold.
+static const char* const kReciprocalFailure = "error: gpu state check failure (stored reciprocal)";
+
+// Walk-completion check (the reference's check.cl:62-73 guard, reported as
+// cl_sieve.cpp:476-480 does): every candidate of a finished segment must have
+// taken every multiplier step of the job's walk.
+static int check_walked(const pfc_job* j, const pfc_ctrl& c) {
+    const uint64_t want = (uint64_t)c.ncand * j->steps_per_cand;
+    if (c.walked == want) return PFC_OK;
+    char msg[160];
+    snprintf(msg, sizeof(msg), "error: gpu validation failure (walked %llu of %llu multiplier steps)",
+             (unsigned long long)c.walked, (unsigned long long)want);
+    return set_error(PFC_ERR_VALIDATION, msg);
+}
+
 // Captures clear -> primes below 114 -> generate -> setup -> walk chunks ->
 // check for a job as one HIP graph; segment bounds come from device memory.
 // Largest multiplier of a job's walk steps below b: n = j, or in P mode the
 // j-th prime of the ascending list.  The walk pairs its steps (tests every
 // second one) on wavefronts whose p all exceed twice this bound.
+// End of a walk launch that would end at b: the job's last step, less the
+// steps the completion-check test hook leaves out (pfc_job_debug_shorten_walk).
+static uint32_t launch_end(const pfc_job* j, uint32_t a, uint32_t b) {
+    if (b != j->jend || j->debug_short == 0) return b;
+    return b - a > j->debug_short ? b - j->debug_short : a;
+}
+
 static uint32_t walk_bound(const pfc_job* j, uint64_t b) {
     if (b == 0) return 0;
     if (j->desc.modes == PFC_MODE_PRIMORIAL) return b <= j->list.size() ? j->list[b - 1] : 0;
@@ -772,7 +797,7 @@ static int build_graph(pfc_job* j) {
                                j->d_rb ? &tb : nullptr, cap, st);
     const uint32_t chunk = std::max<uint32_t>(j->tuning.walk_chunk, 1);
     for (uint64_t a = j->jbegin; a < j->jend && !rc; a += chunk) {
-        const uint32_t b = (uint32_t)std::min<uint64_t>(j->jend, a + chunk);
+        const uint32_t b = launch_end(j, (uint32_t)a, (uint32_t)std::min<uint64_t>(j->jend, a + chunk));
         rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, j->d_clist, j->d_cprefix, d.nmin,
                               j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
                               cap, j->tuning.primes_per_lane, j->tuning.block_size, walk_bound(j, b), st);
@@ -824,15 +849,17 @@ static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
     if (rc) return rc;
     const uint32_t chunk = std::max<uint32_t>(j->tuning.walk_chunk, 1);
     uint64_t launches = 0;
-    (void)hipEventRecord(j->evw0, st);
+    e = hipEventRecord(j->evw0, st);
+    if (e != hipSuccess) return hip_fail(e, "walk timing");
     for (uint64_t a = j->jbegin; a < j->jend; a += chunk, ++launches) {
-        const uint32_t b = (uint32_t)std::min<uint64_t>(j->jend, a + chunk);
+        const uint32_t b = launch_end(j, (uint32_t)a, (uint32_t)std::min<uint64_t>(j->jend, a + chunk));
         rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, j->d_clist, j->d_cprefix, d.nmin,
                               j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
                               cap, j->tuning.primes_per_lane, j->tuning.block_size, walk_bound(j, b), st);
         if (rc) return rc;
     }
-    (void)hipEventRecord(j->evw1, st);
+    e = hipEventRecord(j->evw1, st);
+    if (e != hipSuccess) return hip_fail(e, "walk timing");
     rc = pfc::launch_check(d.modes, d.nmax, j->d_p, j->d_v, j->d_one, j->d_ra, j->d_rb, j->d_ctrl,
                            cap, st);
     if (rc) return rc;
@@ -852,7 +879,7 @@ static int drain_segment(pfc_job* j, bool* overflow, uint64_t launches, bool tim
         *overflow = true;
         return PFC_OK;
     }
-    if (c.flags & 2u) return set_error(PFC_ERR_VALIDATION, "error: gpu validation failure");
+    if (c.flags & 2u) return set_error(PFC_ERR_VALIDATION, kReciprocalFailure);
     if (c.nfactor > j->fcap) {
         // More factor records than the buffer holds (the reference exits here,
         // cl_sieve.cpp:484-487, but its segments are far smaller): the segment
@@ -860,6 +887,8 @@ static int drain_segment(pfc_job* j, bool* overflow, uint64_t launches, bool tim
         *overflow = true;
         return PFC_OK;
     }
+    rc = check_walked(j, c);
+    if (rc) return rc;
     if (c.nfactor) {
         const size_t old = j->factors.size();
         j->factors.resize(old + c.nfactor);
@@ -872,7 +901,8 @@ static int drain_segment(pfc_job* j, bool* overflow, uint64_t launches, bool tim
     j->segments += 1;
     if (timed) {
         float wms = 0;
-        (void)hipEventElapsedTime(&wms, j->evw0, j->evw1);
+        e = hipEventElapsedTime(&wms, j->evw0, j->evw1);
+        if (e != hipSuccess) return hip_fail(e, "walk timing");
         j->walk_ms += wms;
         j->walk_launches += launches;
         j->walk_updates += (uint64_t)c.ncand * j->steps_per_cand;
@@ -883,8 +913,9 @@ static int drain_segment(pfc_job* j, bool* overflow, uint64_t launches, bool tim
 extern "C" int pfc_job_run(pfc_job* j, uint64_t max_segments) {
     pfc::g_err[0] = 0;
     if (!j) return set_error(PFC_ERR_INVALID, "job_run: null job");
-    hipSetDevice(j->dev->ordinal);
-    hipEventRecord(j->ev0, j->stream);
+    hipError_t e = hipSetDevice(j->dev->ordinal);
+    if (e == hipSuccess) e = hipEventRecord(j->ev0, j->stream);
+    if (e != hipSuccess) return hip_fail(e, "job_run start");
     uint64_t done = 0;
     uint64_t span = j->plan.span;
     while (j->pos < j->desc.pmax && (max_segments == 0 || done < max_segments)) {
@@ -909,10 +940,11 @@ extern "C" int pfc_job_run(pfc_job* j, uint64_t max_segments) {
         j->pos = hi;
         ++done;
     }
-    hipEventRecord(j->ev1, j->stream);
-    hipEventSynchronize(j->ev1);
+    e = hipEventRecord(j->ev1, j->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(j->ev1);
     float ms = 0;
-    hipEventElapsedTime(&ms, j->ev0, j->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, j->ev0, j->ev1);
+    if (e != hipSuccess) return hip_fail(e, "job_run end");
     j->device_ms += ms;
     return PFC_OK;
 }
@@ -966,8 +998,8 @@ extern "C" int pfc_job_factors(const pfc_job* j, pfc_factor* out, size_t cap, si
 extern "C" int pfc_job_sync(pfc_job* j) {
     pfc::g_err[0] = 0;
     if (!j) return set_error(PFC_ERR_INVALID, "job_sync: null job");
-    hipSetDevice(j->dev->ordinal);
-    hipError_t e = hipStreamSynchronize(j->stream);
+    hipError_t e = hipSetDevice(j->dev->ordinal);
+    if (e == hipSuccess) e = hipStreamSynchronize(j->stream);
     if (e != hipSuccess) return hip_fail(e, "job_sync");
     return check_tables(j);
 }
@@ -986,12 +1018,25 @@ extern "C" int pfc_job_clear_factors(pfc_job* j) {
     return PFC_OK;
 }
 
+// The job is released even when the device reports an error (its buffers go
+// back to the pool only after its stream has drained); the error is returned.
 extern "C" int pfc_job_destroy(pfc_job* j) {
     if (!j) return PFC_OK;
-    hipSetDevice(j->dev->ordinal);
-    if (j->stream) hipStreamSynchronize(j->stream);
+    pfc::g_err[0] = 0;
+    hipError_t e = hipSetDevice(j->dev->ordinal);
+    if (e == hipSuccess && j->stream) e = hipStreamSynchronize(j->stream);
     free_job_device(j);
     delete j;
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "job_destroy");
+}
+
+extern "C" int pfc_job_debug_shorten_walk(pfc_job* j, uint32_t steps) {
+    if (!j) return set_error(PFC_ERR_INVALID, "debug_shorten_walk: null job");
+    j->debug_short = steps;
+    if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
+    if (j->graph) (void)hipGraphDestroy(j->graph);
+    j->gexec = nullptr;
+    j->graph = nullptr;
     return PFC_OK;
 }
 
@@ -1204,7 +1249,7 @@ int finish_segment(pfc_job* j, const pfc_ctrl& c, uint64_t seg_hi, uint64_t span
     *redo = false;
     int rc = check_tables(j);
     if (rc) return rc;
-    if (c.flags & 2u) return set_error(PFC_ERR_VALIDATION, "error: gpu validation failure");
+    if (c.flags & 2u) return set_error(PFC_ERR_VALIDATION, kReciprocalFailure);
     if (c.nfactor > j->fcap) {
         if (span > 1) {
             *redo = true;
@@ -1214,6 +1259,8 @@ int finish_segment(pfc_job* j, const pfc_ctrl& c, uint64_t seg_hi, uint64_t span
         snprintf(msg, sizeof(msg), "Error: number of results (%u) overflowed array.", c.nfactor);
         return set_error(PFC_ERR_FACTOR_OVERFLOW, msg);
     }
+    rc = check_walked(j, c);
+    if (rc) return rc;
     if (c.nfactor) {
         const size_t old = j->factors.size();
         j->factors.resize(old + c.nfactor);
@@ -1345,6 +1392,7 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
         pfc_job* j = s.job;
         const uint32_t j0 = s.jcur;
         const uint32_t j1 = (uint32_t)std::min<uint64_t>(j->jend, (uint64_t)j0 + max_steps);
+        const uint32_t jw = launch_end(j, j0, j1);          // == j1 unless the test hook is set
         uint32_t flags = (j0 == j->jbegin ? 1u : 0u) | (j1 == j->jend ? 2u : 0u);
         if (s.ncand == 0) {
             finishing[i] = 1;  // nothing to walk; the segment is complete
@@ -1358,7 +1406,7 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
         // per n) plus the start-residue chunk products, each about two steps.
         const double per = (double)(j1 - j0) * ((j->d_rb) ? 1.9 : 1.0) +
                            ((flags & 1u) ? 2.0 * (double)(j->ta.chunks.size() + j->tb.chunks.size()) : 0.0);
-        ranges.push_back({{(uint32_t)i, j0, j1, flags, walk_bound(j, j1), 0, {0, 0}}, count, per});
+        ranges.push_back({{(uint32_t)i, j0, jw, flags, walk_bound(j, jw), 0, {0, 0}}, count, per});
         s.jcur = j1;
         if (j1 == j->jend) finishing[i] = 1;
     }

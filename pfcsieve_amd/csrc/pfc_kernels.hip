@@ -2,7 +2,7 @@
 // C entry points (declared in include/pfc.h).
 //
 //   k_generate : candidate primes of a p-segment (wheel-30 + sieve 7..113 +
-//                base-2 strong PRP), replacing kernels/getsegprimes.cl:464-526
+//                base-2 strong PRP), replacing kernels/getsegprimes.cl:170-232
 //                and kernels/addsmallprimes.cl:21-40.
 //   k_setup    : per-candidate reciprocal, R mod p and start residues
 //                X(nmin-1) mod p, replacing kernels/setup.cl:49-288.
@@ -58,7 +58,7 @@ __device__ __forceinline__ void wave_append(bool keep, uint64_t value, uint64_t*
 // from base30 + b*W*15360 + 60*(w*256 + t); only integers in [lo, hi) are
 // kept.  Survivors of the wheel and the 7..113 sieve are compacted in LDS,
 // then PRP-tested by the whole block (the reference's shape,
-// getsegprimes.cl:477-517, with the LDS list bounded by construction: at most
+// getsegprimes.cl:193-223, with the LDS list bounded by construction: at most
 // 16 survivors per window).  W (1, 2, 4) is the tunable LDS segment size.
 template <int W>
 __device__ __forceinline__ void generate_block(uint64_t base30, uint64_t lo, uint64_t hi,
@@ -130,6 +130,8 @@ __global__ void k_zero_ctrl(pfc_ctrl* ctrl) {
         ctrl->pad = 0;
         ctrl->checksum = 0;
         ctrl->primecount = 0;
+        ctrl->walked = 0;
+        ctrl->reserved = 0;
     }
 }
 
@@ -434,6 +436,20 @@ __device__ __forceinline__ void settle(uint64_t (&r)[K], uint32_t m2, uint32_t m
     }
 }
 
+// Multiplier applications performed by a wavefront, counted in a scalar
+// register where the steps are taken (walk-completion evidence, summed into
+// pfc_ctrl::walked).  The empty asm after each increment makes the count
+// opaque to the optimiser, so it is a real per-trip increment and never a
+// closed form of the loop bounds.
+struct Tally {
+    uint32_t n;
+};
+
+__device__ __forceinline__ void tally(Tally& t, uint32_t k) {
+    t.n += k;
+    asm volatile("" : "+s"(t.n));
+}
+
 struct WalkStream {
     const uint32_t* list;       // P: multipliers
     const uint32_t* composite;  // C, F|C: composite bitmap over n - nbase
@@ -448,14 +464,16 @@ struct WalkStream {
 // One tested step of a chain, alone (no untested step before it).
 template <int K>
 __device__ __forceinline__ void single(uint64_t (&r)[K], uint32_t n, int32_t type, const Lanes<K>& L,
-                                       const WalkStream& W) {
+                                       const WalkStream& W, Tally& T) {
+    tally(T, 1);
     if (step<K>(r, n, L)) settle<K>(r, 0, 0, n, type, L, W.fac, W.fcap, W.ctrl);
 }
 
 // Steps m (untested) and n (tested) of a chain.
 template <int K>
 __device__ __forceinline__ void pair(uint64_t (&r)[K], uint32_t m, uint32_t n, int32_t type,
-                                     const Lanes<K>& L, const WalkStream& W) {
+                                     const Lanes<K>& L, const WalkStream& W, Tally& T) {
+    tally(T, 2);
     mul_step<K>(r, m, L);
     untested<K>(r);
     if (step<K>(r, n, L)) settle<K>(r, 0, m, n, type, L, W.fac, W.fcap, W.ctrl);
@@ -467,7 +485,8 @@ __device__ __forceinline__ void pair(uint64_t (&r)[K], uint32_t m, uint32_t n, i
 // pair(): a hit at m leaves +-n << s, one at n +-1 << s.
 template <int K>
 __device__ __forceinline__ void pair_product(uint64_t (&r)[K], uint32_t m, uint32_t n, int32_t type,
-                                             const Lanes<K>& L, const WalkStream& W) {
+                                             const Lanes<K>& L, const WalkStream& W, Tally& T) {
+    tally(T, 2);
     if (step<K>(r, m * n, L)) settle<K>(r, 0, m, n, type, L, W.fac, W.fcap, W.ctrl);
 }
 
@@ -478,7 +497,8 @@ constexpr uint32_t kProductLimit = 65536;
 // Steps m2, m (untested) and n (tested) of a chain.
 template <int K>
 __device__ __forceinline__ void triple(uint64_t (&r)[K], uint32_t m2, uint32_t m, uint32_t n,
-                                       int32_t type, const Lanes<K>& L, const WalkStream& W) {
+                                       int32_t type, const Lanes<K>& L, const WalkStream& W, Tally& T) {
+    tally(T, 3);
     mul_step<K>(r, m2, L);
     untested<K>(r);
     mul_step<K>(r, m, L);
@@ -491,15 +511,15 @@ __device__ __forceinline__ void triple(uint64_t (&r)[K], uint32_t m2, uint32_t m
 // most kProductLimit take each pair as one step by its product.
 template <int K, int G>
 __device__ __forceinline__ void factorial_steps(uint64_t (&r)[K], const Lanes<K>& L, uint32_t j0,
-                                                uint32_t j1, const WalkStream& W) {
+                                                uint32_t j1, const WalkStream& W, Tally& T) {
     constexpr bool PAIRED = G >= 2;
     uint32_t j = j0;
     if (PAIRED) {
         const uint32_t jp = min(j1, kProductLimit + 1u);
         for (; j + 4 <= jp; j += 4) {
             const uint32_t n0 = __builtin_amdgcn_readfirstlane(j);
-            pair_product<K>(r, n0, n0 + 1, PFC_FACTORIAL, L, W);
-            pair_product<K>(r, n0 + 2, n0 + 3, PFC_FACTORIAL, L, W);
+            pair_product<K>(r, n0, n0 + 1, PFC_FACTORIAL, L, W, T);
+            pair_product<K>(r, n0 + 2, n0 + 3, PFC_FACTORIAL, L, W, T);
         }
     }
     for (; j + 4 <= j1; j += 4) {
@@ -507,14 +527,14 @@ __device__ __forceinline__ void factorial_steps(uint64_t (&r)[K], const Lanes<K>
 #pragma unroll
         for (uint32_t t = 0; t < 4; ++t) n[t] = __builtin_amdgcn_readfirstlane(j + t);
         if (PAIRED) {
-            pair<K>(r, n[0], n[1], PFC_FACTORIAL, L, W);
-            pair<K>(r, n[2], n[3], PFC_FACTORIAL, L, W);
+            pair<K>(r, n[0], n[1], PFC_FACTORIAL, L, W, T);
+            pair<K>(r, n[2], n[3], PFC_FACTORIAL, L, W, T);
         } else {
 #pragma unroll
-            for (uint32_t t = 0; t < 4; ++t) single<K>(r, n[t], PFC_FACTORIAL, L, W);
+            for (uint32_t t = 0; t < 4; ++t) single<K>(r, n[t], PFC_FACTORIAL, L, W, T);
         }
     }
-    for (; j < j1; ++j) single<K>(r, __builtin_amdgcn_readfirstlane(j), PFC_FACTORIAL, L, W);
+    for (; j < j1; ++j) single<K>(r, __builtin_amdgcn_readfirstlane(j), PFC_FACTORIAL, L, W, T);
 }
 
 // Steps by the listed multipliers list[j], j in [j0, j1), in increasing order
@@ -526,13 +546,13 @@ __device__ __forceinline__ void factorial_steps(uint64_t (&r)[K], const Lanes<K>
 // pairs as product steps.
 template <int K, int G>
 __device__ __forceinline__ void list_steps(uint64_t (&r)[K], const uint32_t* list, uint32_t j0, uint32_t j1,
-                                           int32_t type, const Lanes<K>& L, const WalkStream& W) {
+                                           int32_t type, const Lanes<K>& L, const WalkStream& W, Tally& T) {
     constexpr bool PAIRED = G >= 2;
     const ConstU32 lst = (ConstU32)list;
     uint32_t j = j0;
     if ((reinterpret_cast<uintptr_t>(list) & 63u) == 0) {
         const uint32_t jal = min(j1, (j0 + 15u) & ~15u);
-        for (; j < jal; ++j) single<K>(r, __builtin_amdgcn_readfirstlane(lst[j]), type, L, W);
+        for (; j < jal; ++j) single<K>(r, __builtin_amdgcn_readfirstlane(lst[j]), type, L, W, T);
         const ConstU32x16 blocks = (ConstU32x16)list;
         if (G == 3) {
             // Three blocks (48 multipliers) per trip: 16 triples.  What is
@@ -548,7 +568,7 @@ __device__ __forceinline__ void list_steps(uint64_t (&r)[K], const uint32_t* lis
                         const int i = t + u;
                         n[u] = __builtin_amdgcn_readfirstlane(i < 16 ? q0[i] : i < 32 ? q1[i - 16] : q2[i - 32]);
                     }
-                    triple<K>(r, n[0], n[1], n[2], type, L, W);
+                    triple<K>(r, n[0], n[1], n[2], type, L, W, T);
                 }
             }
         }
@@ -559,17 +579,17 @@ __device__ __forceinline__ void list_steps(uint64_t (&r)[K], const uint32_t* lis
 #pragma unroll
                     for (int t = 0; t < 16; t += 2) {
                         const uint32_t m = __builtin_amdgcn_readfirstlane(q[t]);
-                        pair_product<K>(r, m, __builtin_amdgcn_readfirstlane(q[t + 1]), type, L, W);
+                        pair_product<K>(r, m, __builtin_amdgcn_readfirstlane(q[t + 1]), type, L, W, T);
                     }
                 } else {
 #pragma unroll
                     for (int t = 0; t < 16; t += 2)
                         pair<K>(r, __builtin_amdgcn_readfirstlane(q[t]), __builtin_amdgcn_readfirstlane(q[t + 1]),
-                                type, L, W);
+                                type, L, W, T);
                 }
             } else {
 #pragma unroll
-                for (int t = 0; t < 16; ++t) single<K>(r, __builtin_amdgcn_readfirstlane(q[t]), type, L, W);
+                for (int t = 0; t < 16; ++t) single<K>(r, __builtin_amdgcn_readfirstlane(q[t]), type, L, W, T);
             }
         }
     } else {
@@ -579,14 +599,14 @@ __device__ __forceinline__ void list_steps(uint64_t (&r)[K], const uint32_t* lis
             for (int t = 0; t < 16; ++t) q[t] = __builtin_amdgcn_readfirstlane(lst[j + t]);
             if (PAIRED) {
 #pragma unroll
-                for (int t = 0; t < 16; t += 2) pair<K>(r, q[t], q[t + 1], type, L, W);
+                for (int t = 0; t < 16; t += 2) pair<K>(r, q[t], q[t + 1], type, L, W, T);
             } else {
 #pragma unroll
-                for (int t = 0; t < 16; ++t) single<K>(r, q[t], type, L, W);
+                for (int t = 0; t < 16; ++t) single<K>(r, q[t], type, L, W, T);
             }
         }
     }
-    for (; j < j1; ++j) single<K>(r, __builtin_amdgcn_readfirstlane(lst[j]), type, L, W);
+    for (; j < j1; ++j) single<K>(r, __builtin_amdgcn_readfirstlane(lst[j]), type, L, W, T);
 }
 
 // The list index of the first composite at or above n, nbase <= n <= the
@@ -602,19 +622,19 @@ __device__ __forceinline__ uint32_t comp_index(const WalkStream& W, uint32_t n) 
 // G: steps per test (group_steps); 3 is used by P mode only.
 template <int MODE, int K, int G>
 __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K], const Lanes<K>& L,
-                                           uint32_t j0, uint32_t j1, const WalkStream& W) {
+                                           uint32_t j0, uint32_t j1, const WalkStream& W, Tally& T) {
     constexpr bool PAIRED = G >= 2;
     if (MODE == 0) {
-        factorial_steps<K, G>(ra, L, j0, j1, W);
+        factorial_steps<K, G>(ra, L, j0, j1, W, T);
     } else if (MODE == 1) {
-        list_steps<K, G>(ra, W.list, j0, j1, PFC_PRIMORIAL, L, W);
+        list_steps<K, G>(ra, W.list, j0, j1, PFC_PRIMORIAL, L, W, T);
     } else if (W.clist) {
         // F|C (MODE 2) and C (MODE 3) with the composite list: the chains
         // are independent, so F|C walks its factorial chain, then its
         // composite chain, over the same n-range.
-        if (MODE == 2) factorial_steps<K, G>(ra, L, j0, j1, W);
+        if (MODE == 2) factorial_steps<K, G>(ra, L, j0, j1, W, T);
         list_steps<K, G>(MODE == 2 ? rb : ra, W.clist, comp_index(W, j0), comp_index(W, j1), PFC_COMPOSITORIAL,
-                         L, W);
+                         L, W, T);
     } else {
         // F|C (MODE 2) and C (MODE 3) without the list (the stage entry
         // point, and jobs too wide for one): composite flags come from the
@@ -644,22 +664,23 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
 #pragma unroll
                     for (uint32_t t = 0; t < 32; t += 2) {
                         const uint32_t m = __builtin_amdgcn_readfirstlane(nword + t);
-                        pair_product<K>(ra, m, m + 1, PFC_FACTORIAL, L, W);
+                        pair_product<K>(ra, m, m + 1, PFC_FACTORIAL, L, W, T);
                     }
                 } else if (PAIRED && lo == 0u && hi == 32u) {
 #pragma unroll
                     for (uint32_t t = 0; t < 32; t += 2)
                         pair<K>(ra, __builtin_amdgcn_readfirstlane(nword + t),
-                                __builtin_amdgcn_readfirstlane(nword + t + 1), PFC_FACTORIAL, L, W);
+                                __builtin_amdgcn_readfirstlane(nword + t + 1), PFC_FACTORIAL, L, W, T);
                 } else {
                     for (uint32_t n = j; n < jend; ++n)
-                        single<K>(ra, __builtin_amdgcn_readfirstlane(n), PFC_FACTORIAL, L, W);
+                        single<K>(ra, __builtin_amdgcn_readfirstlane(n), PFC_FACTORIAL, L, W, T);
                 }
             }
             if (PAIRED) {
                 if (pend && word) {
                     const uint32_t n = nword + (uint32_t)__builtin_ctz(word);
                     word &= word - 1u;
+                    tally(T, 1);
                     if (step<K>(rc, n, L)) settle<K>(rc, 0, pend, n, PFC_COMPOSITORIAL, L, W.fac, W.fcap, W.ctrl);
                     pend = 0;
                 }
@@ -669,7 +690,7 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
                         word &= word - 1u;
                         const uint32_t n = nword + (uint32_t)__builtin_ctz(word);
                         word &= word - 1u;
-                        pair_product<K>(rc, m, n, PFC_COMPOSITORIAL, L, W);
+                        pair_product<K>(rc, m, n, PFC_COMPOSITORIAL, L, W, T);
                     }
                 } else {
                     while (word & (word - 1u)) {                        // two or more left
@@ -677,11 +698,12 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
                         word &= word - 1u;
                         const uint32_t n = nword + (uint32_t)__builtin_ctz(word);
                         word &= word - 1u;
-                        pair<K>(rc, m, n, PFC_COMPOSITORIAL, L, W);
+                        pair<K>(rc, m, n, PFC_COMPOSITORIAL, L, W, T);
                     }
                 }
                 if (word) {
                     pend = nword + (uint32_t)__builtin_ctz(word);
+                    tally(T, 1);
                     mul_step<K>(rc, pend, L);
                     untested<K>(rc);
                 }
@@ -689,7 +711,7 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
                 while (word) {
                     const uint32_t n = nword + (uint32_t)__builtin_ctz(word);
                     word &= word - 1u;
-                    single<K>(rc, n, PFC_COMPOSITORIAL, L, W);
+                    single<K>(rc, n, PFC_COMPOSITORIAL, L, W, T);
                 }
             }
             j = jend;
@@ -701,18 +723,33 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
 
 // Walks steps [j0, j1) in groups of as many steps per test as the wavefront
 // allows (triples in P mode only).
+// Returns the multiplier applications the wavefront performed.
 template <int MODE, int K>
-__device__ __forceinline__ void walk_range(uint64_t (&ra)[K], uint64_t (&rb)[K], Lanes<K>& L,
-                                           uint32_t j0, uint32_t j1, uint32_t bmax, const WalkStream& W) {
+__device__ __forceinline__ uint32_t walk_range(uint64_t (&ra)[K], uint64_t (&rb)[K], Lanes<K>& L,
+                                               uint32_t j0, uint32_t j1, uint32_t bmax, const WalkStream& W) {
+    Tally T{0};
     const int g = group_steps<K>(L, bmax, MODE == 1);
     if constexpr (MODE == 1) {
         if (g == 3) {
-            walk_steps<MODE, K, 3>(ra, rb, L, j0, j1, W);
-            return;
+            walk_steps<MODE, K, 3>(ra, rb, L, j0, j1, W, T);
+            return T.n;
         }
     }
-    if (g >= 2) walk_steps<MODE, K, 2>(ra, rb, L, j0, j1, W);
-    else walk_steps<MODE, K, 1>(ra, rb, L, j0, j1, W);
+    if (g >= 2) walk_steps<MODE, K, 2>(ra, rb, L, j0, j1, W, T);
+    else walk_steps<MODE, K, 1>(ra, rb, L, j0, j1, W, T);
+    return T.n;
+}
+
+// Adds the wavefront's multiplier applications times its live candidates
+// to ctrl->walked (one 64-bit atomic from the first active lane).
+template <int K>
+__device__ __forceinline__ void add_walked(uint32_t taken, const Lanes<K>& L, pfc_ctrl* ctrl) {
+    uint32_t live = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) live += (uint32_t)__popcll(__ballot(L.live[k]));
+    const uint64_t act = __ballot(true);
+    if ((int)__lane_id() == __ffsll((unsigned long long)act) - 1)
+        atomicAdd((unsigned long long*)&ctrl->walked, (unsigned long long)taken * live);
 }
 
 __host__ __device__ constexpr int walk_mode(uint32_t modes) {
@@ -742,7 +779,8 @@ __global__ __launch_bounds__(256) void k_walk(const WalkArgs A) {
         rb[k] = (MODE == 2) ? A.RB[src] : 0;
     }
     const WalkStream W{A.list, A.composite, A.clist, A.cprefix, A.nbase, A.fac, A.fcap, A.ctrl};
-    walk_range<MODE, K>(ra, rb, L, A.j0, A.j1, A.bmax, W);
+    const uint32_t taken = walk_range<MODE, K>(ra, rb, L, A.j0, A.j1, A.bmax, W);
+    add_walked<K>(taken, L, A.ctrl);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (!L.live[k]) continue;
@@ -841,12 +879,14 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
         }
     }
     const WalkStream W{J.list, J.composite, J.clist, J.cprefix, J.nmin, J.fac, J.fcap, J.ctrl};
+    uint32_t taken;
     switch (walk_mode(J.modes)) {
-        case 0: walk_range<0, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
-        case 1: walk_range<1, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
-        case 2: walk_range<2, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
-        default: walk_range<3, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
+        case 0: taken = walk_range<0, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
+        case 1: taken = walk_range<1, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
+        case 2: taken = walk_range<2, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
+        default: taken = walk_range<3, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
     }
+    add_walked<K>(taken, L, J.ctrl);
     uint64_t sum = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -1197,6 +1237,11 @@ int pfc_stage_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_l
     pfc::g_err[0] = 0;
     if (!d_p || !d_v || !d_res_a || !d_ctrl || (!d_fac && fcap))
         return pfc::set_error(PFC_ERR_INVALID, "walk: null pointer");
+    // The 3-by-2 step takes any 64-bit residue only for multipliers below
+    // 2^31 (an untested paired step may leave its result unreduced).
+    if (j1 > 0x7FFFFFFFu) return pfc::set_error(PFC_ERR_INVALID, "walk: j1 > 2^31 - 1");
+    if (modes != PFC_MODE_PRIMORIAL && j0 < 2 && j0 < j1)
+        return pfc::set_error(PFC_ERR_INVALID, "walk: multiplier steps start below 2");
     // The multiplier of step j is j except in P mode, whose list this call
     // cannot bound without reading it: P steps are then each tested alone.
     const uint32_t bmax = (modes == PFC_MODE_PRIMORIAL || j1 <= j0) ? 0 : j1 - 1;

@@ -193,6 +193,14 @@ def setup_table_size(kind: int, N: int):
     return nch.value, nb.value
 
 
+def setup_table_cost(kind: int, N: int) -> int:
+    """Modular products one candidate's start residue X(N) costs in the
+    set-up: a product per table chunk plus the squarings between exponent
+    bit groups."""
+    nch, nb = setup_table_size(kind, N)
+    return nch + max(nb - 1, 0)
+
+
 def format_factor(f: Factor) -> str:
     buf = C.create_string_buffer(96)
     lib().pfc_format_factor(C.byref(f), buf, 96)

@@ -6,6 +6,8 @@ import ctypes as C
 import numpy as np
 import pytest
 
+from pfcsieve_amd._native import CTRL_BYTES
+
 pytestmark = pytest.mark.gpu
 
 F, P, Cm = 1, 2, 4
@@ -20,7 +22,7 @@ def gen_on_gpu(dev, lo, hi, cap=None):
     if cap is None:
         cap = int(2.2 * span / max(np.log(max(span, 3)), 1.0)) + 4096
     d_p = DeviceBuffer(dev, cap * 8)
-    d_ctrl = DeviceBuffer(dev, 32)
+    d_ctrl = DeviceBuffer(dev, CTRL_BYTES)
     d_ctrl.zero()
     N.check(N.lib().pfc_stage_generate(lo, hi, d_p.ptr, cap, d_ctrl.ptr, None))
     N.check(N.lib().pfc_device_synchronize(dev.handle))
@@ -197,7 +199,7 @@ def test_stages_individually(orc, dev, modes, nmin, nmax):
     cap = 8192
     bufs = {k: DeviceBuffer(dev, cap * w) for k, w in
             [("p", 8), ("v", 4), ("one", 8), ("ra", 8), ("rb", 8)]}
-    ctrl = DeviceBuffer(dev, 32)
+    ctrl = DeviceBuffer(dev, CTRL_BYTES)
     ctrl.zero()
     fac = DeviceBuffer(dev, 16 * 1024)
     Nt.check(L.pfc_stage_generate(lo, hi, bufs["p"].ptr, cap, ctrl.ptr, None))
@@ -236,6 +238,9 @@ def test_stages_individually(orc, dev, modes, nmin, nmax):
     c = np.frombuffer(ctrl.download(), np.uint64)
     ref = orc.run(modes, lo, hi, nmin, nmax)
     assert int(c[2]) == ref.dev_checksum and int(c[3]) == ref.primecount == n
+    # walk-completion count: every candidate took every multiplier step
+    steps = (nmax - nmin if modes & F else 0) + ((nmax - nmin) - len(isp) if modes & Cm else 0)
+    assert int(c[4]) == n * steps
 
 
 @pytest.mark.gpu
@@ -253,7 +258,7 @@ def test_primorial_walk_stage(orc, dev, offset):
     lo, hi = 10**10, 10**10 + 50000
     cap = 8192
     bufs = {k: DeviceBuffer(dev, cap * w) for k, w in [("p", 8), ("v", 4), ("one", 8), ("ra", 8)]}
-    ctrl = DeviceBuffer(dev, 32)
+    ctrl = DeviceBuffer(dev, CTRL_BYTES)
     ctrl.zero()
     fac = DeviceBuffer(dev, 16 * 1024)
     Nt.check(L.pfc_stage_generate(lo, hi, bufs["p"].ptr, cap, ctrl.ptr, None))
@@ -278,6 +283,7 @@ def test_primorial_walk_stage(orc, dev, offset):
     ref = orc.run(P, lo, hi, nmin, nmax)
     assert int(c[2]) == ref.dev_checksum and int(c[3]) == ref.primecount == int(u[0])
     assert int(u[1]) == ref.raw_factors
+    assert int(c[4]) == int(u[0]) * len(primes)        # walk-completion count
 
 
 def _x_of(orc, kind, N, p):
@@ -395,7 +401,7 @@ def test_full_size_walk_sampled_against_oracle(orc, dev, modes, lo, width, nmin,
     cap = 1 << 19
     two = modes == (F | Cm)
     bufs = {k: DeviceBuffer(dev, cap * w) for k, w in [("p", 8), ("v", 4), ("one", 8), ("ra", 8), ("rb", 8)]}
-    ctrl = DeviceBuffer(dev, 32)
+    ctrl = DeviceBuffer(dev, CTRL_BYTES)
     ctrl.zero()
     fac = DeviceBuffer(dev, 16 * 4096)
     Nt.check(L.pfc_stage_generate(lo, lo + width, bufs["p"].ptr, cap, ctrl.ptr, None))
@@ -597,7 +603,7 @@ def test_walk_launches_end_at_hits(orc, dev, modes):
     cap = 1 << 15
     two = modes == (F | Cm)
     bufs = {k: DeviceBuffer(dev, cap * w) for k, w in [("p", 8), ("v", 4), ("one", 8), ("ra", 8), ("rb", 8)]}
-    ctrl = DeviceBuffer(dev, 32)
+    ctrl = DeviceBuffer(dev, CTRL_BYTES)
     ctrl.zero()
     fcap = 1 << 14
     fac = DeviceBuffer(dev, 16 * fcap)
@@ -625,6 +631,8 @@ def test_walk_launches_end_at_hits(orc, dev, modes):
     u = np.frombuffer(ctrl.download(), np.uint32)
     assert int(c[2]) == ref.dev_checksum and int(c[3]) == ref.primecount == int(u[0])
     assert int(u[1]) == ref.raw_factors and ref.raw_factors <= fcap
+    steps = (nmax - nmin if modes & F else 0) + ((nmax - nmin) - len(isp) if modes & Cm else 0)
+    assert int(c[4]) == int(u[0]) * steps                # walk-completion count over all launches
     raw = np.frombuffer(fac.download(16 * int(u[1])), dtype=orc.FACTOR_DTYPE)
     got = sorted((int(r["p"]), int(r["nc"]), int(r["type"])) for r in raw if sympy.isprime(int(r["p"])))
     assert got == sorted((int(r["p"]), int(r["nc"]), int(r["type"])) for r in ref.factors)
