@@ -146,13 +146,7 @@ def roofline(head, mul_rates):
     cand = head["candidates_timed"]
     achieved = mads * cand / launches / avg_s
     ups = head["walk_updates_per_s"]
-    # VALU issue slots per update from the instruction sequence: a step is 8
-    # single-rate instructions (5 multiply-adds, the 64-bit add, the quotient
-    # compare, the exec-masked correction add) and 2 double-rate register
-    # moves; a pair of steps adds the band test (an add and a compare); a
-    # product step covers two updates with one step and its test.
-    f_prod = prod / upd
-    slots = (1 - f_prod) * (8 + 1 + 1) + f_prod * (8 + 1 + 2) / 2
+    mad_rate = mul_rates["mad_u64_u32"]
     return {
         "bound": "valu-int32", "kernel": "k_walk on the headline job",
         "achieved": achieved / 1e12, "peak": PEAK_LANE_OPS / 1e12, "unit": "Tops/s",
@@ -160,10 +154,10 @@ def roofline(head, mul_rates):
         "traffic": (WALK_FETCH_PER_CAND + WALK_WRITE_PER_CAND) * cand,
         "mads_per_update": mads / upd, "updates_per_launch": upd * cand / launches,
         "avg_launch_ms": avg_s * 1e3,
-        "valu_issue": {"slots_per_update": slots, "achieved": ups * slots / 1e12,
-                       "frac": ups * slots / PEAK_LANE_OPS,
-                       "note": "single-rate instruction slots per update of the walk's instruction sequence "
-                               "(double-rate moves at half a slot) x updates/s, against the same peak"},
+        "frac_of_measured_mad_rate": achieved / mad_rate,
+        "issue": "the walk is bound by VALU issue, not by its multiplies: 5 v_mad_u64_u32 per update share "
+                 "the issue slots with the 64-bit add, the quotient compare and correction add, the band "
+                 "test and two register moves (rocprof VALU and SALU counts per update in profiles/)",
         "reference_equivalent": {"mul_per_update": W_MUL_REF, "achieved": ups * W_MUL_REF / 1e12,
                                  "frac": ups * W_MUL_REF / PEAK_LANE_OPS,
                                  "note": "updates/s priced at the reference's 19 INT32 multiplies per update "

@@ -252,12 +252,22 @@ int pfc_device_synchronize(pfc_device* dev);
 int pfc_stage_generate(uint64_t lo, uint64_t hi, uint64_t* d_p, uint32_t cap, pfc_ctrl* d_ctrl,
                        void* stream);
 
-/* Host-built start-residue tables (replacing cl_sieve.cpp:842-1326): chunks
- * are products of primes below 2^63, grouped by exponent bit, high bit first. */
+/* Host-built start-residue tables (replacing cl_sieve.cpp:842-1326).  The
+ * exponents e(r) of the primes r <= N in X(N) are written in base B = 2^k
+ * (k = digit_bits, chosen per table by the host to minimise the modular
+ * products): group (b, t) holds the primes whose exponent digit at position b
+ * is t, multiplied into chunks below 2^63.  Groups are stored position-major
+ * from the highest position, digits from B-1 down to 1, so group
+ * g = (npos-1-b)*(B-1) + (B-1-t) is chunks[groups[g] .. groups[g+1]), and
+ *   X = prod_b (prod_t D_{b,t}^t)^(B^b)
+ * with D_{b,t} the product of group (b, t).  The set-up evaluates it by
+ * Horner's rule over positions, each H_b = prod_t D_{b,t}^t by suffix
+ * products (every chunk multiplied once, one more product per digit). */
 typedef struct pfc_setup_table {
     const uint64_t* chunks;    /* device */
-    const uint32_t* groups;    /* device: nbits+1 offsets into chunks, group b = bit b */
-    uint32_t nbits;
+    const uint32_t* groups;    /* device: npos*(2^digit_bits - 1) + 1 offsets into chunks */
+    uint32_t npos;             /* digit positions of the largest exponent (0: X = 1) */
+    uint32_t digit_bits;       /* k, 1..8 */
 } pfc_setup_table;
 
 /* (b) Per candidate: reciprocal, Montgomery one (2^64 mod p) and start
@@ -297,12 +307,15 @@ int pfc_stage_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const ui
                     const uint64_t* d_one, const uint64_t* d_res_a, const uint64_t* d_res_b,
                     pfc_ctrl* d_ctrl, uint32_t cap, void* stream);
 
-/* Builds the setup table for X(N) on the host: kind 0 = N!, 1 = N#, 2 = N!/N#.
- * Writes up to chunk_cap chunks and nbits+1 group offsets (group_cap >= 33):
- * X = prod over b of (prod of chunks[groups[b] .. groups[b+1]))^(2^b), which
- * the set-up evaluates by Horner's rule over b, high bit first. */
+/* Builds the setup table for X(N) on the host: kind 0 = N!, 1 = N#, 2 = N!/N#
+ * (layout as pfc_setup_table).  Writes up to chunk_cap chunks and group_cap
+ * group offsets and reports the full sizes in *nchunks and *ngroups
+ * (= npos*(2^digit_bits - 1) + 1), the positions in *npos and the digit
+ * width in *digit_bits; with chunks == NULL and groups == NULL only the sizes
+ * are returned. */
 int pfc_build_setup_table(uint32_t kind, uint32_t N, uint64_t* chunks, size_t chunk_cap,
-                          size_t* nchunks, uint32_t* groups, size_t group_cap, uint32_t* nbits);
+                          size_t* nchunks, uint32_t* groups, size_t group_cap, size_t* ngroups,
+                          uint32_t* npos, uint32_t* digit_bits);
 
 /* Start-table self-check (verifyslow.cl, verify.cl, verifyresult.cl;
  * cl_sieve.cpp:917-970, 1079-1134, 1246-1304): X(N) of a table of `kind`

@@ -6,5 +6,5 @@ tests, the benchmark and the serving loop.
 """
 from ._native import PfcError, lib  # noqa: F401
 from .sieve import (COMPOSITORIAL, FACTORIAL, MODE_COMPOSITORIAL, MODE_FACTORIAL,  # noqa: F401
-                    MODE_PRIMORIAL, PRIMORIAL, Batcher, Device, Job, SieveResult, footprint,
+                    MODE_PRIMORIAL, PRIMORIAL, Batcher, Device, Job, SieveResult, TableLayout, footprint,
                     job_result, run_batch, run_sieve, setup_table_cost, setup_table_size)

@@ -60,7 +60,8 @@ assert C.sizeof(Ctrl) == CTRL_BYTES
 
 
 class SetupTable(C.Structure):
-    _fields_ = [("chunks", C.c_void_p), ("groups", C.c_void_p), ("nbits", C.c_uint32)]
+    _fields_ = [("chunks", C.c_void_p), ("groups", C.c_void_p), ("npos", C.c_uint32),
+                ("digit_bits", C.c_uint32)]
 
 
 # Every symbol include/pfc.h declares, with (restype, argtypes).
@@ -109,7 +110,7 @@ SIGNATURES = {
     "pfc_stage_check": (_int, [_u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
     "pfc_stage_verify_table": (_int, [_u32, _u32, C.POINTER(SetupTable), _vp, _vp]),
     "pfc_build_setup_table": (_int, [_u32, _u32, C.POINTER(_u64), _sz, C.POINTER(_sz),
-                                     C.POINTER(_u32), _sz, C.POINTER(_u32)]),
+                                     C.POINTER(_u32), _sz, C.POINTER(_sz), C.POINTER(_u32), C.POINTER(_u32)]),
     "pfc_probe_mul_rates": (_int, [_vp, C.POINTER(C.c_double * 3)]),
     "pfc_probe_copy": (_int, [_vp, _u64, C.POINTER(C.c_double)]),
 }

@@ -300,28 +300,59 @@ PFC_HD bool is_prime_u32(uint32_t n) {
 
 // ---------------------------------------------------------------- start residues
 
-// Start residue (X mod p) << s of X = prod over groups b (high first) of
-// (prod of chunks in group b)^(2^b), by Horner's rule in the shifted
-// representation, chunks as plain integers below 2^63: the chunk products
-// (mulmod64_nofix) accept the unreduced accumulator, which is reduced only
-// before each squaring and at the end.  With first/stride, only the chunks
-// first, first + stride, ... of each group are taken: the product over
-// first = 0 .. stride - 1 of those partial values is the full one, which
-// lets a workgroup share the work.
+// Start residue (X mod p) << s of X = prod over primes r <= N of r^e(r), from
+// a table that writes the exponents in base B = 2^k (k = digit_bits, 1..8):
+// group (b, t) holds the chunks (products of primes below 2^63) of the primes
+// whose exponent digit at position b is t, and
+//   X = prod_b H_b^(B^b),   H_b = prod_{t=1..B-1} D_{b,t}^t,
+// with D_{b,t} the product of group (b, t).  Horner's rule over the positions,
+// high first, takes acc <- acc^B * H_b, and H_b costs each chunk once plus
+// one product per digit by the suffix products S_t = prod_{t' >= t} D_{b,t'}:
+// H_b = prod_t S_t.  Groups are stored position-major from the highest
+// position, digits from B-1 down to 1: group g = (npos-1-b)*(B-1) + (B-1-t),
+// chunks groups[g] .. groups[g+1].  For k = 1 this is the plain binary
+// Horner scheme (acc <- acc^2 * D_b), evaluated without the suffix product.
+//
+// Chunk products (mulmod64_nofix) accept the unreduced accumulator, which is
+// reduced only before a squaring or a residue-by-residue product and at the
+// end.  With first/stride, only the chunks first, first + stride, ... of each
+// group are taken: X is multiplicative in the chunks for every exponent
+// layout, so the product over first = 0 .. stride - 1 of those partial values
+// is the full one, which lets a workgroup share the work.
 template <class Chunks, class Groups>
-PFC_HD uint64_t start_residue(Chunks chunks, Groups groups, uint32_t nbits, const Modulus& m,
+PFC_HD uint64_t start_residue(Chunks chunks, Groups groups, uint32_t npos, uint32_t k, const Modulus& m,
                               uint32_t first = 0, uint32_t stride = 1) {
     const uint64_t d = mod_d(m);
-    uint64_t acc = 1ull << m.s;                                     // 1, shifted
-    for (int b = (int)nbits - 1; b >= 0; --b) {
-        if (b != (int)nbits - 1) {
-            if (acc >= d) acc -= d;
-            acc = mulmod64(acc, acc >> m.s, m);
+    const uint64_t one = 1ull << m.s;                               // 1, shifted
+    const uint32_t ndig = (1u << k) - 1;
+    uint64_t acc = one;
+    uint32_t g = 0;
+    bool fix;
+    for (int b = (int)npos - 1; b >= 0; --b) {
+        if (b != (int)npos - 1) {
+            for (uint32_t i = 0; i < k; ++i) {
+                if (acc >= d) acc -= d;
+                acc = mulmod64(acc, acc >> m.s, m);
+            }
         }
-        const uint32_t g0 = groups[b], g1 = groups[b + 1];
-        for (uint32_t i = g0 + first; i < g1; i += stride) {
-            bool fix;
-            acc = mulmod64_nofix(acc, chunks[i], m, &fix);
+        if (ndig == 1) {
+            const uint32_t g0 = groups[g], g1 = groups[g + 1];
+            for (uint32_t i = g0 + first; i < g1; i += stride) acc = mulmod64_nofix(acc, chunks[i], m, &fix);
+            ++g;
+            continue;
+        }
+        uint64_t S = one;
+        bool started = false;
+        for (uint32_t t = 0; t < ndig; ++t, ++g) {
+            const uint32_t g0 = groups[g], g1 = groups[g + 1];
+            for (uint32_t i = g0 + first; i < g1; i += stride) S = mulmod64_nofix(S, chunks[i], m, &fix);
+            started |= g1 > g0;
+            if (started) {
+                // both canonical: acc * (S >> s) < d * p <= d * 2^64
+                if (S >= d) S -= d;
+                if (acc >= d) acc -= d;
+                acc = mulmod64_nofix(acc, S >> m.s, m, &fix);
+            }
         }
     }
     return acc >= d ? acc - d : acc;

@@ -70,13 +70,15 @@ std::vector<uint32_t> primes_in(uint64_t lo, uint64_t hi) {
 }
 
 // Setup table (include/pfc.h pfc_setup_table): exponents of every prime r <= N
-// in X(N) = N! (kind 0), N# (kind 1) or N!/N# (kind 2); group b holds the
-// primes with bit b of the exponent set, multiplied into chunks below 2^63
-// (so that a chunk product accepts an unreduced accumulator).
+// in X(N) = N! (kind 0), N# (kind 1) or N!/N# (kind 2), written in base
+// 2^digit_bits; group (b, t) holds the primes with digit t at position b,
+// multiplied into chunks below 2^63 (so that a chunk product accepts an
+// unreduced accumulator).
 struct HostTable {
     std::vector<uint64_t> chunks;
-    std::vector<uint32_t> groups;  // nbits + 1 offsets
-    uint32_t nbits = 0;
+    std::vector<uint32_t> groups;  // npos * (2^digit_bits - 1) + 1 offsets
+    uint32_t npos = 0;             // digit positions of the largest exponent
+    uint32_t digit_bits = 1;
     uint32_t kind = 0, N = 0;      // X(N) = N!, N# or N!/N#
 };
 
@@ -84,6 +86,40 @@ constexpr u128 kChunkLimit = (u128)1 << 63;
 // Widest n-range (nmax - nmin) whose composites are also listed for the walk
 // (4 bytes each); wider C and F|C ranges walk the bitmap alone.
 constexpr uint32_t kMaxCompositeList = 1u << 27;
+constexpr uint32_t kMaxDigitBits = 8;
+
+uint32_t positions_of(uint32_t maxe, uint32_t k) {
+    uint32_t n = 0;
+    while (n * k < 32 && (maxe >> (n * k))) ++n;
+    return n;
+}
+
+// Modular products the set-up spends on a table of digit width k, estimated
+// from the bits of each group (about 60 of a chunk's 63 are filled), plus the
+// suffix products (the largest non-empty digit of each position, k > 1) and
+// the squarings between positions.
+double table_cost(const std::vector<uint32_t>& pr, const std::vector<uint32_t>& ex,
+                  const std::vector<float>& lg, uint32_t maxe, uint32_t k) {
+    const uint32_t npos = positions_of(maxe, k), B = 1u << k;
+    std::vector<double> bits((size_t)npos * B, 0.0);
+    std::vector<uint32_t> top(npos, 0);
+    for (size_t i = 0; i < pr.size(); ++i) {
+        for (uint32_t b = 0; b < npos; ++b) {
+            const uint32_t t = (ex[i] >> (b * k)) & (B - 1);
+            if (!t) {
+                if ((ex[i] >> (b * k)) == 0) break;
+                continue;
+            }
+            bits[(size_t)b * B + t] += lg[i];
+            top[b] = std::max(top[b], t);
+        }
+    }
+    double cost = (double)k * (npos ? npos - 1 : 0);
+    for (double x : bits) cost += ceil(x / 60.0);
+    if (k > 1)
+        for (uint32_t t : top) cost += t;
+    return cost;
+}
 
 HostTable build_table(uint32_t kind, uint32_t N) {
     HostTable T;
@@ -91,6 +127,7 @@ HostTable build_table(uint32_t kind, uint32_t N) {
     T.N = N;
     std::vector<uint32_t> pr = primes_in(2, N);
     std::vector<uint32_t> ex(pr.size());
+    std::vector<float> lg(pr.size());
     uint32_t maxe = 0;
     for (size_t i = 0; i < pr.size(); ++i) {
         uint32_t e = 1;
@@ -100,31 +137,51 @@ HostTable build_table(uint32_t kind, uint32_t N) {
             if (kind == 2) e -= 1;
         }
         ex[i] = e;
+        lg[i] = log2f((float)pr[i]);
         maxe = std::max(maxe, e);
     }
-    T.nbits = maxe ? 32 - __builtin_clz(maxe) : 0;
-    T.groups.assign(T.nbits + 1, 0);
+    // Digit width: the cheapest by the estimate (ties to the smaller).
+    uint32_t k = 1;
+    if (maxe > 1) {
+        double best = table_cost(pr, ex, lg, maxe, 1);
+        for (uint32_t kk = 2; kk <= kMaxDigitBits; ++kk) {
+            const double c = table_cost(pr, ex, lg, maxe, kk);
+            if (c < best) {
+                best = c;
+                k = kk;
+            }
+        }
+    }
+    T.digit_bits = k;
+    T.npos = positions_of(maxe, k);
+    const uint32_t B = 1u << k, ndig = B - 1;
+    // Buckets in group order (position-major from the highest position,
+    // digits from B-1 down), each filled in increasing prime order.
+    std::vector<std::vector<uint32_t>> bucket((size_t)T.npos * ndig);
+    for (size_t i = 0; i < pr.size(); ++i) {
+        for (uint32_t b = 0; b < T.npos; ++b) {
+            const uint32_t rest = ex[i] >> (b * k);
+            if (!rest) break;
+            const uint32_t t = rest & (B - 1);
+            if (t) bucket[(size_t)(T.npos - 1 - b) * ndig + (B - 1 - t)].push_back(pr[i]);
+        }
+    }
+    T.groups.assign(bucket.size() + 1, 0);
     // Chunks are filled largest-first: the largest remaining primes while they
     // fit, then the smallest remaining ones as filler (about 4% fewer chunks
     // than filling in increasing order; the group's product is unchanged).
-    std::vector<uint32_t> g;
-    // Exponents fall with the prime (e(r) <= N / (r - 1)), so bit b >= 1 can be
-    // set only for r <= N / 2^b + 1: each group scans the primes up to there.
-    for (uint32_t b = 0; b < T.nbits; ++b) {
-        T.groups[b] = (uint32_t)T.chunks.size();
-        g.clear();
-        const uint64_t rmax = b ? (uint64_t)N / (1ull << b) + 1 : UINT64_MAX;
-        for (size_t i = 0; i < pr.size() && pr[i] <= rmax; ++i)
-            if ((ex[i] >> b) & 1u) g.push_back(pr[i]);          // ascending
-        size_t lo = 0, hi = g.size();
+    for (size_t g = 0; g < bucket.size(); ++g) {
+        T.groups[g] = (uint32_t)T.chunks.size();
+        const std::vector<uint32_t>& v = bucket[g];
+        size_t lo = 0, hi = v.size();
         while (lo < hi) {
-            uint64_t acc = g[--hi];
-            while (lo < hi && (u128)acc * g[hi - 1] < kChunkLimit) acc *= g[--hi];
-            while (lo < hi && (u128)acc * g[lo] < kChunkLimit) acc *= g[lo++];
+            uint64_t acc = v[--hi];
+            while (lo < hi && (u128)acc * v[hi - 1] < kChunkLimit) acc *= v[--hi];
+            while (lo < hi && (u128)acc * v[lo] < kChunkLimit) acc *= v[lo++];
             T.chunks.push_back(acc);
         }
     }
-    if (T.nbits) T.groups[T.nbits] = (uint32_t)T.chunks.size();
+    T.groups[bucket.size()] = (uint32_t)T.chunks.size();
     return T;
 }
 
@@ -632,7 +689,7 @@ int queue_table_check(pfc_job* j) {
     const uint32_t* gs[2] = {j->d_ga, j->d_gb};
     const int n = two ? 2 : 1;
     for (int i = 0; i < n; ++i) {
-        const pfc_setup_table t{cs[i], gs[i], hs[i]->nbits};
+        const pfc_setup_table t{cs[i], gs[i], hs[i]->npos, hs[i]->digit_bits};
         uint64_t* work = j->d_verify + (size_t)i * PFC_VERIFY_WORDS;
         int rc = pfc::launch_verify_table(hs[i]->kind, hs[i]->N, &t, work, j->stream);
         if (rc) return rc;
@@ -790,8 +847,8 @@ static int build_graph(pfc_job* j) {
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
     int rc = pfc::launch_head_dev(j->d_bounds, j->plan.span, j->d_p, cap, j->d_ctrl, st, j->tuning.lds_segment);
-    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits};
-    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits};
+    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.npos, j->ta.digit_bits};
+    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.npos, j->tb.digit_bits};
     if (!rc)
         rc = pfc::launch_setup(j->d_p, j->d_ctrl, j->d_v, j->d_one, j->d_ra, j->d_rb, &ta,
                                j->d_rb ? &tb : nullptr, cap, st);
@@ -842,8 +899,8 @@ static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
     if (e != hipSuccess) return hip_fail(e, "ctrl clear");
     int rc = pfc::launch_generate(lo, hi, j->d_p, cap, j->d_ctrl, st, j->tuning.lds_segment);
     if (rc) return rc;
-    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits};
-    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits};
+    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.npos, j->ta.digit_bits};
+    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.npos, j->tb.digit_bits};
     rc = pfc::launch_setup(j->d_p, j->d_ctrl, j->d_v, j->d_one, j->d_ra, j->d_rb, &ta,
                            j->d_rb ? &tb : nullptr, cap, st);
     if (rc) return rc;
@@ -1072,8 +1129,8 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
     hipError_t e = hipMemsetAsync(j->d_ctrl, 0, sizeof(pfc_ctrl), st);
     if (e != hipSuccess) rc = hip_fail(e, "autotune clear");
     if (!rc) rc = pfc::launch_generate(d.pmin, d.pmin + j->plan.span, j->d_p, cap, j->d_ctrl, st, t.lds_segment);
-    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.nbits};
-    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.nbits};
+    pfc_setup_table ta{j->d_ca, j->d_ga, j->ta.npos, j->ta.digit_bits};
+    pfc_setup_table tb{j->d_cb, j->d_gb, j->tb.npos, j->tb.digit_bits};
     if (!rc)
         rc = pfc::launch_setup(j->d_p, j->d_ctrl, j->d_v, j->d_one, j->d_ra, j->d_rb, &ta,
                                j->d_rb ? &tb : nullptr, cap, st);
@@ -1214,12 +1271,8 @@ pfc_batch_job batch_view(pfc_job* j) {
     b.ctrl = j->d_ctrl;
     b.fac = j->d_fac;
     b.fcap = (uint32_t)j->fcap;
-    b.nba = j->ta.nbits;
-    b.nbb = j->tb.nbits;
-    b.ca = j->d_ca;
-    b.ga = j->d_ga;
-    b.cb = j->d_cb;
-    b.gb = j->d_gb;
+    b.ta = pfc_setup_table{j->d_ca, j->d_ga, j->ta.npos, j->ta.digit_bits};
+    b.tb = pfc_setup_table{j->d_cb, j->d_gb, j->tb.npos, j->tb.digit_bits};
     b.list = j->d_list;
     b.composite = j->d_comp;
     b.clist = j->d_clist;
@@ -1681,12 +1734,15 @@ extern "C" int pfc_parse_uint64(const char* s, uint64_t lo, uint64_t hi, uint64_
 // ====================================================================== tables (C ABI)
 
 extern "C" int pfc_build_setup_table(uint32_t kind, uint32_t N, uint64_t* chunks, size_t chunk_cap,
-                                     size_t* nchunks, uint32_t* groups, size_t group_cap,
-                                     uint32_t* nbits) {
-    if (kind > 2 || !nchunks || !nbits) return set_error(PFC_ERR_INVALID, "build_setup_table: bad args");
+                                     size_t* nchunks, uint32_t* groups, size_t group_cap, size_t* ngroups,
+                                     uint32_t* npos, uint32_t* digit_bits) {
+    if (kind > 2 || !nchunks || !ngroups || !npos || !digit_bits)
+        return set_error(PFC_ERR_INVALID, "build_setup_table: bad args");
     HostTable T = build_table(kind, N);
     *nchunks = T.chunks.size();
-    *nbits = T.nbits;
+    *ngroups = T.groups.size();
+    *npos = T.npos;
+    *digit_bits = T.digit_bits;
     if (chunks) memcpy(chunks, T.chunks.data(), std::min(chunk_cap, T.chunks.size()) * 8);
     if (groups) memcpy(groups, T.groups.data(), std::min(group_cap, T.groups.size()) * 4);
     return PFC_OK;

@@ -19,11 +19,9 @@ struct pfc_batch_job {
     uint64_t* RB;
     pfc_ctrl* ctrl;
     pfc_factor* fac;
-    uint32_t fcap, nba, nbb, pad;
-    const uint64_t* ca;
-    const uint32_t* ga;
-    const uint64_t* cb;
-    const uint32_t* gb;
+    uint32_t fcap, pad;
+    pfc_setup_table ta;        // start table of res_a
+    pfc_setup_table tb;        // start table of res_b (F|C)
     const uint32_t* list;
     const uint32_t* composite;
     const uint32_t* clist;     // C, F|C (optional): composites in [nmin, nmax), increasing

@@ -160,19 +160,15 @@ typedef const uint64_t __attribute__((address_space(4)))* ConstU64;
 typedef uint32_t U32x16 __attribute__((ext_vector_type(16)));
 typedef const U32x16 __attribute__((address_space(4)))* ConstU32x16;
 
-__device__ __forceinline__ uint64_t horner_start(const uint64_t* chunks, const uint32_t* groups,
-                                                 uint32_t nbits, const Modulus& m) {
-    return pfc::start_residue((ConstU64)chunks, (ConstU32)groups, nbits, m);
+__device__ __forceinline__ uint64_t horner_start(const pfc_setup_table& t, const Modulus& m) {
+    return pfc::start_residue((ConstU64)t.chunks, (ConstU32)t.groups, t.npos, t.digit_bits, m);
 }
 
 __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
                                                const pfc_ctrl* __restrict__ ctrl,
                                                uint32_t* __restrict__ V, uint64_t* __restrict__ ONE,
                                                uint64_t* __restrict__ RA, uint64_t* __restrict__ RB,
-                                               const uint64_t* __restrict__ ca,
-                                               const uint32_t* __restrict__ ga, uint32_t nba,
-                                               const uint64_t* __restrict__ cb,
-                                               const uint32_t* __restrict__ gb, uint32_t nbb, uint32_t cap) {
+                                               const pfc_setup_table ta, const pfc_setup_table tb, uint32_t cap) {
     uint32_t n = ctrl->ncand;
     if (n > cap) n = cap;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -180,8 +176,8 @@ __global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P,
         const Modulus m = pfc::make_modulus(p);
         V[i] = m.v;
         ONE[i] = pfc::mont_one(m);
-        RA[i] = horner_start(ca, ga, nba, m);
-        if (RB) RB[i] = horner_start(cb, gb, nbb, m);
+        RA[i] = horner_start(ta, m);
+        if (RB) RB[i] = horner_start(tb, m);
     }
 }
 
@@ -223,14 +219,14 @@ __global__ __launch_bounds__(256) void k_verify_slow(uint32_t kind, uint32_t N, 
 constexpr uint32_t kVerifyLanes = 256;
 static_assert(PFC_VERIFY_BLOCKS == kVerifyLanes, "one slow-product partial per lane");
 
-__global__ __launch_bounds__(kVerifyLanes) void k_verify_finish(const uint64_t* __restrict__ chunks,
-                                                                const uint32_t* __restrict__ groups,
-                                                                uint32_t nbits, uint64_t* __restrict__ work) {
+__global__ __launch_bounds__(kVerifyLanes) void k_verify_finish(const pfc_setup_table tab,
+                                                                uint64_t* __restrict__ work) {
     const Modulus m = pfc::make_modulus(kTestPrime);     // s = 0: residues are plain
     const uint32_t t = threadIdx.x;
     __shared__ uint64_t sx[kVerifyLanes], st[kVerifyLanes];
     sx[t] = work[2 + t];
-    st[t] = pfc::start_residue((ConstU64)chunks, (ConstU32)groups, nbits, m, t, kVerifyLanes);
+    st[t] = pfc::start_residue((ConstU64)tab.chunks, (ConstU32)tab.groups, tab.npos, tab.digit_bits, m, t,
+                               kVerifyLanes);
     __syncthreads();
     for (uint32_t h = kVerifyLanes / 2; h > 0; h >>= 1) {
         if (t < h) {
@@ -869,8 +865,8 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
             const Modulus m = pfc::make_modulus(p);
             mone[k] = pfc::mont_one(m);
             lanes_from_state<K>(L, p, m.v, k);
-            ra[k] = horner_start(J.ca, J.ga, J.nba, m);
-            rb[k] = two ? horner_start(J.cb, J.gb, J.nbb, m) : 0;
+            ra[k] = horner_start(J.ta, m);
+            rb[k] = two ? horner_start(J.tb, m) : 0;
         } else {
             mone[k] = 0;
             lanes_from_state<K>(L, p, J.V[src], k);
@@ -1063,9 +1059,9 @@ int launch_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, uin
     uint32_t blocks = (cap + 255) / 256;
     if (blocks == 0) blocks = 1;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, st, d_p, d_ctrl, d_v, d_one, d_ra, d_rb,
-                       a->chunks, a->groups, a->nbits, b ? b->chunks : nullptr,
-                       b ? b->groups : nullptr, b ? b->nbits : 0u, cap);
+    const pfc_setup_table none{nullptr, nullptr, 0, 1};
+    hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, st, d_p, d_ctrl, d_v, d_one, d_ra, d_rb, *a,
+                       b ? *b : none, cap);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_setup launch");
 }
@@ -1148,8 +1144,7 @@ int launch_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uin
     hipLaunchKernelGGL(k_verify_slow, dim3(PFC_VERIFY_BLOCKS), dim3(256), 0, st, kind, N, d_work);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "k_verify_slow launch");
-    hipLaunchKernelGGL(k_verify_finish, dim3(1), dim3(kVerifyLanes), 0, st, t->chunks, t->groups, t->nbits,
-                       d_work);
+    hipLaunchKernelGGL(k_verify_finish, dim3(1), dim3(kVerifyLanes), 0, st, *t, d_work);
     e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_verify_finish launch");
 }
@@ -1226,6 +1221,8 @@ int pfc_stage_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, 
     pfc::g_err[0] = 0;
     if (!d_p || !d_ctrl || !d_v || !d_one || !d_res_a)
         return pfc::set_error(PFC_ERR_INVALID, "setup: null pointer");
+    if ((a && (a->digit_bits < 1 || a->digit_bits > 8)) || (b && (b->digit_bits < 1 || b->digit_bits > 8)))
+        return pfc::set_error(PFC_ERR_INVALID, "setup: digit_bits must be 1..8");
     return pfc::launch_setup(d_p, d_ctrl, d_v, d_one, d_res_a, d_res_b, a, b, cap,
                              (hipStream_t)stream);
 }
@@ -1267,8 +1264,10 @@ int pfc_stage_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, 
     if (!t || !d_work) return pfc::set_error(PFC_ERR_INVALID, "verify_table: null pointer");
     if (kind > 2) return pfc::set_error(PFC_ERR_INVALID, "verify_table: kind must be 0, 1 or 2");
     if (N > 0x7FFFFFFFu) return pfc::set_error(PFC_ERR_INVALID, "verify_table: N >= 2^31");
-    if (t->nbits && (!t->chunks || !t->groups))
+    if (t->npos && (!t->chunks || !t->groups))
         return pfc::set_error(PFC_ERR_INVALID, "verify_table: table pointers missing");
+    if (t->digit_bits < 1 || t->digit_bits > 8)
+        return pfc::set_error(PFC_ERR_INVALID, "verify_table: digit_bits must be 1..8");
     return pfc::launch_verify_table(kind, N, t, d_work, (hipStream_t)stream);
 }
 

@@ -185,20 +185,67 @@ def footprint(dev: Optional[Device], modes, pmin, pmax, nmin, nmax) -> int:
     return b.value
 
 
+class TableLayout:
+    """A start-residue table built on the host (pfc_build_setup_table):
+    chunks, group offsets, digit positions and digit width."""
+
+    def __init__(self, kind: int, N: int):
+        nch, ng, npos, k = C.c_size_t(), C.c_size_t(), C.c_uint32(), C.c_uint32()
+        check(lib().pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(ng), C.byref(npos),
+                                          C.byref(k)))
+        ch = (C.c_uint64 * max(nch.value, 1))()
+        gr = (C.c_uint32 * ng.value)()
+        check(lib().pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, ng.value, C.byref(ng),
+                                          C.byref(npos), C.byref(k)))
+        self.kind, self.N = kind, N
+        self.chunks = list(ch[: nch.value])
+        self.groups = list(gr)
+        self.npos, self.digit_bits = npos.value, k.value
+
+    def group(self, b: int, t: int):
+        """Chunks of the primes whose exponent digit at position b is t."""
+        nd = (1 << self.digit_bits) - 1
+        g = (self.npos - 1 - b) * nd + (nd - t)
+        return self.chunks[self.groups[g]: self.groups[g + 1]]
+
+    def value(self) -> int:
+        """X(N) from the table (big integer), by the set-up's own evaluation
+        order: Horner over positions, suffix products over digits."""
+        B = 1 << self.digit_bits
+        x = 1
+        for b in range(self.npos - 1, -1, -1):
+            x = x ** B
+            S, h = 1, 1
+            for t in range(B - 1, 0, -1):
+                for c in self.group(b, t):
+                    S *= c
+                h *= S
+            x *= h
+        return x
+
+    def products(self) -> int:
+        """Modular products one candidate's start residue costs: every chunk,
+        the suffix products from each position's largest non-empty digit
+        down (none for digit width 1), and the squarings between positions."""
+        k, B = self.digit_bits, 1 << self.digit_bits
+        n = len(self.chunks) + k * max(self.npos - 1, 0)
+        if k > 1:
+            for b in range(self.npos):
+                top = max((t for t in range(1, B) if self.group(b, t)), default=0)
+                n += top
+        return n
+
+
 def setup_table_size(kind: int, N: int):
-    """(chunks, exponent bits) of the start-residue table for X(N) with
-    kind 0 = N!, 1 = N#, 2 = N!/N# (host only)."""
-    nch, nb = C.c_size_t(), C.c_uint32()
-    check(lib().pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb)))
-    return nch.value, nb.value
+    """(chunks, digit positions, digit bits) of the start-residue table for
+    X(N) with kind 0 = N!, 1 = N#, 2 = N!/N# (host only)."""
+    t = TableLayout(kind, N)
+    return len(t.chunks), t.npos, t.digit_bits
 
 
 def setup_table_cost(kind: int, N: int) -> int:
-    """Modular products one candidate's start residue X(N) costs in the
-    set-up: a product per table chunk plus the squarings between exponent
-    bit groups."""
-    nch, nb = setup_table_size(kind, N)
-    return nch + max(nb - 1, 0)
+    """Modular products one candidate's start residue X(N) costs in the set-up."""
+    return TableLayout(kind, N).products()
 
 
 def format_factor(f: Factor) -> str:

@@ -27,8 +27,18 @@ uint64_t t_mont_one(uint64_t p) { return pfc::mont_one(pfc::make_modulus(p)); }
 int t_modulus_ok(uint64_t p) { return pfc::modulus_ok(p, pfc::make_modulus(p)); }
 int t_strong_prp2(uint64_t n) { return pfc::strong_prp2(n); }
 int t_is_prime_u32(uint32_t n) { return pfc::is_prime_u32(n); }
-uint64_t t_start_residue(const uint64_t* chunks, const uint32_t* groups, uint32_t nbits, uint64_t p) {
-    return pfc::start_residue(chunks, groups, nbits, pfc::make_modulus(p));
+uint64_t t_start_residue(const uint64_t* chunks, const uint32_t* groups, uint32_t npos, uint32_t k, uint64_t p) {
+    return pfc::start_residue(chunks, groups, npos, k, pfc::make_modulus(p));
+}
+// The same with the chunks shared over `stride` lanes: the product of the
+// partial values of lanes 0 .. stride-1 (the table self-check's split).
+uint64_t t_start_residue_split(const uint64_t* chunks, const uint32_t* groups, uint32_t npos, uint32_t k,
+                               uint64_t p, uint32_t stride) {
+    const pfc::Modulus m = pfc::make_modulus(p);
+    uint64_t acc = 1ull << m.s;
+    for (uint32_t t = 0; t < stride; ++t)
+        acc = pfc::mulmod64(acc, pfc::start_residue(chunks, groups, npos, k, m, t, stride) >> m.s, m);
+    return acc;
 }
 uint64_t t_to_mont(uint64_t p, uint64_t x_shifted) {
     const pfc::Modulus m = pfc::make_modulus(p);

@@ -171,21 +171,18 @@ def _table(dev, kind, N, corrupt=False):
     """pfc_build_setup_table on the host, uploaded to device buffers
     (corrupt: the first chunk is changed before the upload)."""
     import pfcsieve_amd._native as Nt
-    from pfcsieve_amd.sieve import DeviceBuffer
-    L = Nt.lib()
-    nch, nb = C.c_size_t(), C.c_uint32()
-    Nt.check(L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb)))
-    ch = (C.c_uint64 * max(nch.value, 1))()
-    gr = (C.c_uint32 * (nb.value + 1))()
-    Nt.check(L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb)))
+    from pfcsieve_amd.sieve import DeviceBuffer, TableLayout
+    t = TableLayout(kind, N)
+    ch = (C.c_uint64 * max(len(t.chunks), 1))(*t.chunks)
+    gr = (C.c_uint32 * len(t.groups))(*t.groups)
     if corrupt:
-        assert nch.value > 0
+        assert t.chunks
         ch[0] = ch[0] + 2 if ch[0] < 2**62 else ch[0] - 2
-    dc = DeviceBuffer(dev, 8 * max(nch.value, 1))
+    dc = DeviceBuffer(dev, 8 * max(len(t.chunks), 1))
     dc.upload(bytes(ch))
-    dg = DeviceBuffer(dev, 4 * (nb.value + 1))
+    dg = DeviceBuffer(dev, 4 * len(t.groups))
     dg.upload(bytes(gr))
-    return Nt.SetupTable(dc.ptr.value, dg.ptr.value, nb.value), (dc, dg)
+    return Nt.SetupTable(dc.ptr.value, dg.ptr.value, t.npos, t.digit_bits), (dc, dg)
 
 
 @pytest.mark.parametrize("modes,nmin,nmax", [(F, 2000, 2600), (Cm, 2000, 2600), (F | Cm, 2000, 2600)])
@@ -314,7 +311,7 @@ def test_table_self_check(orc, dev, kind, N):
     Nt.check(L.pfc_device_synchronize(dev.handle))
     got = np.frombuffer(work.download(16), np.uint64)
     assert int(got[0]) == want and int(got[1]) == want
-    if t.nbits:
+    if t.npos:
         tb, keep_b = _table(dev, kind, N, corrupt=True)
         Nt.check(L.pfc_stage_verify_table(kind, N, C.byref(tb), work.ptr, None))
         Nt.check(L.pfc_device_synchronize(dev.handle))

@@ -119,20 +119,55 @@ def test_verify_factor(L):
 @pytest.mark.parametrize("kind,N", [(0, 100), (0, 4999), (0, 100000), (1, 100), (1, 99999),
                                     (2, 100), (2, 77777)])
 def test_setup_table_reconstructs_start_value(L, kind, N):
-    nch, nb = C.c_size_t(), C.c_uint32()
-    L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb))
-    ch = (C.c_uint64 * nch.value)()
-    gr = (C.c_uint32 * (nb.value + 1))()
-    assert L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb)) == 0
-    x = 1
-    for b in range(nb.value - 1, -1, -1):      # Horner over exponent bits, high first
-        x = x * x
-        for k in range(gr[b], gr[b + 1]):
-            x *= ch[k]
+    """The table's documented evaluation (Horner over base-2^k digit
+    positions, suffix products over digits) gives X(N) exactly."""
+    from pfcsieve_amd.sieve import TableLayout
+    t = TableLayout(kind, N)
     prim = sympy.primorial(N, nth=False)
     expect = [math.factorial(N), prim, math.factorial(N) // prim][kind]
-    assert x == expect
-    assert all(1 < c < 2**63 for c in ch)
+    assert t.value() == expect
+    assert all(1 < c < 2**63 for c in t.chunks)
+    assert 1 <= t.digit_bits <= 8 and len(t.groups) == t.npos * ((1 << t.digit_bits) - 1) + 1
+
+
+@pytest.mark.parametrize("kind,N", [(0, 3000), (0, 20000), (2, 20000), (1, 5000)])
+def test_setup_table_exponents_are_legendre(L, kind, N):
+    """Factoring every chunk back into primes: a prime r in group (b, t)
+    contributes t * B^b to its exponent, and the sum over groups is its
+    exponent in X(N) (Legendre's formula; minus one for N!/N#; one for N#)."""
+    from pfcsieve_amd.sieve import TableLayout
+    t = TableLayout(kind, N)
+    B = 1 << t.digit_bits
+    got = {}
+    for b in range(t.npos):
+        for d in range(1, B):
+            for c in t.group(b, d):
+                for r, mult in sympy.factorint(c).items():
+                    assert mult == 1 and r <= N               # distinct primes within a chunk
+                    got[r] = got.get(r, 0) + d * B ** b
+    want = {}
+    for r in sympy.primerange(2, N + 1):
+        e = 1 if kind == 1 else sum(N // r ** i for i in range(1, 40) if r ** i <= N) - (kind == 2)
+        if e:
+            want[r] = e
+    assert got == want
+
+
+@pytest.mark.parametrize("kind,N,ratio", [(0, 999999, 0.8), (2, 999999, 0.75), (0, 99999, 0.9)])
+def test_setup_table_digit_width_saves_products(L, kind, N, ratio):
+    """At large N the chosen digit width needs fewer modular products than
+    binary digits (each prime multiplied in popcount(e) times): 999999! from
+    34,892 to about 26,200."""
+    from pfcsieve_amd.sieve import TableLayout
+    t = TableLayout(kind, N)
+    assert t.digit_bits > 1
+    # the binary layout's cost: every prime once per set exponent bit
+    pr = list(sympy.primerange(2, N + 1))
+    binary_bits = 0
+    for r in pr:
+        e = sum(N // r ** i for i in range(1, 40) if r ** i <= N) - (kind == 2)
+        binary_bits += bin(e).count("1") * math.log2(r)
+    assert t.products() < ratio * binary_bits / 60
 
 
 # ----------------------------------------------------------------- device arithmetic on the host
@@ -160,8 +195,10 @@ def A():
     L.t_is_prime_u32.argtypes = [u32]
     L.t_generate.argtypes = [u64, u64, C.POINTER(u64), C.c_size_t]
     L.t_generate.restype = C.c_size_t
-    L.t_start_residue.argtypes = [C.POINTER(u64), C.POINTER(u32), u32, u64]
+    L.t_start_residue.argtypes = [C.POINTER(u64), C.POINTER(u32), u32, u32, u64]
     L.t_start_residue.restype = u64
+    L.t_start_residue_split.argtypes = [C.POINTER(u64), C.POINTER(u32), u32, u32, u64, u32]
+    L.t_start_residue_split.restype = u64
     L.t_to_mont.argtypes = [u64, u64]
     L.t_to_mont.restype = u64
     L.t_walk_step.argtypes = [u64, u64, u32, C.POINTER(u64), C.POINTER(C.c_int)]
@@ -321,17 +358,20 @@ def test_generate_window_logic_matches_oracle(A, orc, lo, hi):
 
 @pytest.mark.parametrize("kind,N", [(0, 100), (0, 4999), (0, 99999), (1, 100), (1, 77777), (2, 100), (2, 65432)])
 def test_start_residue_against_bigint(L, A, kind, N):
-    """Horner's rule over the table in the shifted representation gives
-    (X mod p) << s for X = N!, N#, N!/N#."""
-    nch, nb = C.c_size_t(), C.c_uint32()
-    L.pfc_build_setup_table(kind, N, None, 0, C.byref(nch), None, 0, C.byref(nb))
-    ch = (C.c_uint64 * max(nch.value, 1))()
-    gr = (C.c_uint32 * (nb.value + 1))()
-    L.pfc_build_setup_table(kind, N, ch, nch.value, C.byref(nch), gr, nb.value + 1, C.byref(nb))
+    """The set-up's evaluation of the table in the shifted representation
+    gives (X mod p) << s for X = N!, N#, N!/N#, whole and split over 256
+    lanes as the table self-check splits it."""
+    from pfcsieve_amd.sieve import TableLayout
+    t = TableLayout(kind, N)
+    ch = (C.c_uint64 * max(len(t.chunks), 1))(*t.chunks)
+    gr = (C.c_uint32 * len(t.groups))(*t.groups)
     prim = sympy.primorial(N, nth=False)
     X = [math.factorial(N), prim, math.factorial(N) // prim][kind]
     rng = random.Random(kind * 1000 + N)
     for p in _primes_mix(rng, 40) + [101, 103, 10007, N + 1 if (N + 1) % 2 else N + 2]:
         s = 64 - p.bit_length()
-        got = A.t_start_residue(ch, gr, nb.value, p)
+        got = A.t_start_residue(ch, gr, t.npos, t.digit_bits, p)
         assert got == (X % p) << s, (kind, N, p)
+    for p in _primes_mix(rng, 5):
+        s = 64 - p.bit_length()
+        assert A.t_start_residue_split(ch, gr, t.npos, t.digit_bits, p, 256) == (X % p) << s
