@@ -108,13 +108,30 @@ PFC_HD uint64_t mulmod32_nofix(uint64_t a, uint32_t n, uint32_t k1, uint32_t k0,
     return rem3by2_nofix(hi, (uint32_t)lo, k1, k0, nd, d, v, fix);
 }
 
-// The walk's test for a result x of mulmod32_nofix, with one = 2^s (the
-// shifted 1), c = ~one and thr = d - 2*one - 1:  t = x + c = x - one - 1
-// (mod 2^64) is at least thr exactly when x <= one (t wraps) or x >= d - one.
-// That covers x == 1 and x == -1 (one and d - one) and every unreduced x
-// (x >= d) with one add and one compare; the only other values it passes are
-// the 2^(s+1) - 1 integers x < one or d - one < x < d, a fraction 2/p.
-PFC_HD bool walk_attention(uint64_t x, uint64_t c, uint64_t thr) { return x + c >= thr; }
+// The walk's test for a result x of mulmod32_nofix against a band B (B = 1
+// shifted, 2^s, or B = b << s for a group of steps): it must pass every x
+// with x <= B or x >= d - B (a hit, or the residue after a hit at an untested
+// step) and every unreduced x (x >= d).  It looks at the high word alone:
+// with bh = B >> 32 and dh = (d - B) >> 32, x <= B implies x_hi <= bh and
+// x >= d - B implies x_hi >= dh, so
+//   x_hi + c >= thr  (mod 2^32),  c = ~bh = -bh - 1,  thr = dh - bh - 1,
+// (x_hi - bh - 1 wraps for x_hi <= bh, and is at least thr exactly when
+// x_hi >= dh) passes all of them with one 32-bit add and one 32-bit compare.
+// It also passes the x with x_hi == bh above B or x_hi == dh below d - B, a
+// fraction below 2^33 / d of the residues, which the slow path re-tests
+// exactly.  When dh <= bh the band covers the high words of [B, d - B)
+// entirely and the constants make every x pass (c = thr = 0).
+PFC_HD void walk_band(uint64_t d, uint64_t B, uint32_t* c, uint32_t* thr) {
+    const uint32_t bh = (uint32_t)(B >> 32), dh = (uint32_t)((d - B) >> 32);
+    if (dh > bh) {
+        *c = ~bh;
+        *thr = dh - bh - 1;
+    } else {
+        *c = 0;
+        *thr = 0;
+    }
+}
+PFC_HD bool walk_attention(uint64_t x, uint32_t c, uint32_t thr) { return (uint32_t)(x >> 32) + c >= thr; }
 
 // The constants k1, k0 of mulmod32_nofix for d = <d1, d0>.
 PFC_HD uint32_t walk_k1(uint32_t d1, uint32_t d0) { return d0 ? ~d1 : 0u - d1; }

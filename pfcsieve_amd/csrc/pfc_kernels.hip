@@ -285,7 +285,8 @@ struct WalkArgs {
 // Residues are plain shifted forms x = X << s, so +1 and -1 are one = 2^s and
 // d - one.  A step leaves x == X or x == X + d (the rare second correction of
 // the 3-by-2 division not applied); pfc::walk_attention finds both the +-1
-// residues and the unreduced ones with one add and one compare.
+// residues and the unreduced ones with one 32-bit add and one 32-bit compare
+// on the residue's high word.
 //
 // Paired steps.  When every multiplier of a launch is at most bmax and every
 // p of a wavefront exceeds 2 * bmax, only every second step of a chain is
@@ -306,8 +307,9 @@ struct WalkArgs {
 template <int K>
 struct Lanes {
     uint32_t k1[K], k0[K], v[K], s[K];    // k1, k0: pfc::walk_k1 / walk_k0 of d
-    uint64_t d[K], nd[K], c[K], thr[K];   // nd = 2^64 - d; test: c = ~b, thr = d - 2*b - 1
-    bool live[K];                         //   with b = one, or b = B for paired steps
+    uint32_t c[K], thr[K];                // test constants, pfc::walk_band of d and the band
+    uint64_t d[K], nd[K];                 // nd = 2^64 - d
+    bool live[K];
 };
 
 template <int K>
@@ -318,9 +320,7 @@ __device__ __forceinline__ void lanes_from_state(Lanes<K>& L, uint64_t p, uint32
     L.k0[k] = pfc::walk_k0((uint32_t)L.d[k]);
     L.nd[k] = 0ull - L.d[k];
     L.v[k] = v;
-    const uint64_t one = 1ull << L.s[k];
-    L.c[k] = ~one;
-    L.thr[k] = L.d[k] - 2 * one - 1;
+    pfc::walk_band(L.d[k], 1ull << L.s[k], &L.c[k], &L.thr[k]);    // band: +-1
 }
 
 // Steps per test the wavefront may take for multipliers up to bmax (0 =
@@ -341,8 +341,7 @@ __device__ __forceinline__ int group_steps(Lanes<K>& L, uint32_t bmax, bool trip
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const uint64_t B = (g == 3 ? (uint64_t)bmax * bmax : (uint64_t)bmax) << L.s[k];
-        L.c[k] = ~B;
-        L.thr[k] = L.d[k] - 2 * B - 1;
+        pfc::walk_band(L.d[k], B, &L.c[k], &L.thr[k]);
     }
     return g;
 }

@@ -54,13 +54,17 @@ uint64_t t_walk_step(uint64_t p, uint64_t a_shifted, uint32_t n, uint64_t* raw, 
     uint64_t x = pfc::mulmod32_nofix(a_shifted, n, pfc::walk_k1(m.d1, m.d0), pfc::walk_k0(m.d0), 0ull - d,
                                      d, m.v, &fix);
     *raw = x;
-    *attention = pfc::walk_attention(x, ~one, d - 2 * one - 1);
+    uint32_t c, thr;
+    pfc::walk_band(d, one, &c, &thr);
+    *attention = pfc::walk_attention(x, c, thr);
     return x >= d ? x - d : x;
 }
-int t_walk_attention(uint64_t p, uint64_t x) {
+// The walk's test of x against the band B (B = 2^s: +-1).
+int t_walk_attention(uint64_t p, uint64_t B, uint64_t x) {
     const pfc::Modulus m = pfc::make_modulus(p);
-    const uint64_t d = pfc::mod_d(m), one = 1ull << m.s;
-    return pfc::walk_attention(x, ~one, d - 2 * one - 1);
+    uint32_t c, thr;
+    pfc::walk_band(pfc::mod_d(m), B, &c, &thr);
+    return pfc::walk_attention(x, c, thr);
 }
 // Candidates of [lo, hi) (lo >= 114) exactly as k_generate's lanes compute them.
 size_t t_generate(uint64_t lo, uint64_t hi, uint64_t* out, size_t cap) {

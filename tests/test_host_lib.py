@@ -203,7 +203,7 @@ def A():
     L.t_to_mont.restype = u64
     L.t_walk_step.argtypes = [u64, u64, u32, C.POINTER(u64), C.POINTER(C.c_int)]
     L.t_walk_step.restype = u64
-    L.t_walk_attention.argtypes = [u64, u64]
+    L.t_walk_attention.argtypes = [u64, u64, u64]
     L.t_mulmod64_nofix.argtypes = [u64, u64, u64, C.POINTER(C.c_int)]
     L.t_mulmod64_nofix.restype = u64
     return L
@@ -263,16 +263,30 @@ def test_mulmod64_unreduced_input(A):
 
 
 def test_walk_attention_bounds(A):
-    """The walk's one-compare test passes exactly the x with x <= 2^s or
-    x >= d - 2^s, checked at both edges of both intervals and at random x."""
+    """The walk's high-word test against a band B passes every x <= B and
+    every x >= d - B (which includes the unreduced x >= d), and passes an x
+    strictly between only when its high word equals that of B or of d - B
+    (then exactly those); checked at both edges of both intervals and at
+    random x, for B = 2^s (+-1) and for paired and triple bands b << s, up to
+    bands too wide for the high word, where every x passes."""
     rng = random.Random(11)
     for p in _primes_mix(rng, 500):
         s = 64 - p.bit_length()
         d, one = p << s, 1 << s
-        edges = [0, 1, one - 1, one, one + 1, d - one - 1, d - one, d - one + 1, d - 1, d, d + 1, R - 1]
-        for x in edges + [rng.randrange(R) for _ in range(20)] + [rng.randrange(d) for _ in range(20)]:
-            x %= R
-            assert A.t_walk_attention(p, x) == int(x <= one or x >= d - one), (p, x)
+        bands = [one, rng.randrange(1, max(2, p // 2)) << s, (p // 2 - 1) << s if p > 4 else one]
+        for B in bands:
+            bh, dh = B >> 32, (d - B) >> 32
+            edges = [0, 1, B - 1, B, B + 1, d - B - 1, d - B, d - B + 1, d - 1, d, d + 1, R - 1,
+                     (bh << 32) | 0xFFFFFFFF, dh << 32]
+            for x in edges + [rng.randrange(R) for _ in range(20)] + [rng.randrange(d) for _ in range(20)]:
+                x %= R
+                got = A.t_walk_attention(p, B, x)
+                if x <= B or x >= d - B:
+                    assert got == 1, (p, B, x)
+                elif dh > bh:
+                    assert got == int((x >> 32) in (bh, dh)), (p, B, x)
+                else:
+                    assert got == 1, (p, B, x)
 
 
 def test_walk_step_against_bigint(A):
