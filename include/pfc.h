@@ -334,6 +334,22 @@ int pfc_build_setup_table(uint32_t kind, uint32_t N, uint64_t* chunks, size_t ch
 int pfc_stage_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_work,
                            void* stream);
 
+/* The same self-check without a workspace argument and without allocating:
+ * no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The block
+ * partials live in a library-owned device array of PFC_NOALLOC_SCRATCH_BYTES
+ * bytes per GPU, held until the process ends (its size does not depend on
+ * kind, N or the table, so no valid input exceeds it); the results go to
+ * d_out[0] (Mont(X) from the product) and d_out[1] (the table's start
+ * residue), as d_work[0..1] of pfc_stage_verify_table.  Calls on one GPU
+ * share that array: calls that overlap on one GPU (other streams or
+ * threads) race, so a caller must order them.  No hipMalloc, hipFree or other
+ * allocation is made; the array's device address is looked up once per GPU
+ * on the first call there. */
+#define PFC_NOALLOC_SCRATCH_BYTES (8 * PFC_VERIFY_WORDS)
+int pfc_stage_verify_table_noalloc(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_out,
+                                   void* stream);
+
 /* ------------------------------------------------------------------ probe */
 /* Measures the lane-op rate of independent v_mad_u64_u32 / v_mul_lo_u32 /
  * v_mul_hi_u32 streams on all CUs (the INT32 multiply roofline).  rates[3]

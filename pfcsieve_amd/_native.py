@@ -109,6 +109,7 @@ SIGNATURES = {
                               _u32, _u32, _vp]),
     "pfc_stage_check": (_int, [_u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
     "pfc_stage_verify_table": (_int, [_u32, _u32, C.POINTER(SetupTable), _vp, _vp]),
+    "pfc_stage_verify_table_noalloc": (_int, [_u32, _u32, C.POINTER(SetupTable), _vp, _vp]),
     "pfc_build_setup_table": (_int, [_u32, _u32, C.POINTER(_u64), _sz, C.POINTER(_sz),
                                      C.POINTER(_u32), _sz, C.POINTER(_sz), C.POINTER(_u32), C.POINTER(_u32)]),
     "pfc_probe_mul_rates": (_int, [_vp, C.POINTER(C.c_double * 3)]),

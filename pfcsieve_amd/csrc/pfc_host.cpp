@@ -691,7 +691,7 @@ int queue_table_check(pfc_job* j) {
     for (int i = 0; i < n; ++i) {
         const pfc_setup_table t{cs[i], gs[i], hs[i]->npos, hs[i]->digit_bits};
         uint64_t* work = j->d_verify + (size_t)i * PFC_VERIFY_WORDS;
-        int rc = pfc::launch_verify_table(hs[i]->kind, hs[i]->N, &t, work, j->stream);
+        int rc = pfc::launch_verify_table(hs[i]->kind, hs[i]->N, &t, work, work, j->stream);
         if (rc) return rc;
         e = hipMemcpyAsync(j->h_verify + 2 * i, work, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, j->stream);
         if (e != hipSuccess) return hip_fail(e, "table check");

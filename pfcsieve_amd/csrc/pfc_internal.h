@@ -69,8 +69,10 @@ int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint3
                  pfc_ctrl* d_ctrl, uint32_t cap, hipStream_t st);
 int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, uint32_t nranges,
                  uint32_t* d_head, uint32_t ppl, uint32_t nblocks, hipStream_t st);
+// d_work: PFC_VERIFY_WORDS words (block partials in [2, PFC_VERIFY_WORDS));
+// the two results go to d_out[0], d_out[1] (d_out may be d_work).
 int launch_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_work,
-                        hipStream_t st);
+                        uint64_t* d_out, hipStream_t st);
 int probe_mul_rates(int cus, double rates[3]);
 int probe_copy(uint64_t bytes, double* ms);
 

@@ -20,12 +20,22 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/pfc.h"
 #include "pfc_arith.h"
 #include "pfc_internal.h"
 
 using pfc::Modulus;
+
+// Device scratch of pfc_stage_verify_table_noalloc: the slow product's block
+// partials (and the unused result words of the workspace layout), 2,064
+// bytes, held by every GPU the library runs on until the process ends.  In
+// host code `scratch` names a host-side stand-in, not the device array: host
+// code uses only the address hipGetSymbolAddress returns for the current
+// device (scratch_address below).
+static __device__ __align__(256) uint64_t scratch[PFC_VERIFY_WORDS];
+static_assert(sizeof(scratch) == PFC_NOALLOC_SCRATCH_BYTES, "scratch size as declared in pfc.h");
 
 namespace {
 
@@ -220,7 +230,8 @@ constexpr uint32_t kVerifyLanes = 256;
 static_assert(PFC_VERIFY_BLOCKS == kVerifyLanes, "one slow-product partial per lane");
 
 __global__ __launch_bounds__(kVerifyLanes) void k_verify_finish(const pfc_setup_table tab,
-                                                                uint64_t* __restrict__ work) {
+                                                                const uint64_t* __restrict__ work,
+                                                                uint64_t* __restrict__ out) {
     const Modulus m = pfc::make_modulus(kTestPrime);     // s = 0: residues are plain
     const uint32_t t = threadIdx.x;
     __shared__ uint64_t sx[kVerifyLanes], st[kVerifyLanes];
@@ -237,8 +248,8 @@ __global__ __launch_bounds__(kVerifyLanes) void k_verify_finish(const pfc_setup_
     }
     if (t == 0) {
         const uint64_t one = pfc::mont_one(m);
-        work[0] = pfc::to_mont(sx[0], one, m);           // X * R mod p
-        work[1] = pfc::to_mont(st[0], one, m);
+        out[0] = pfc::to_mont(sx[0], one, m);            // X * R mod p
+        out[1] = pfc::to_mont(st[0], one, m);
     }
 }
 
@@ -1139,13 +1150,35 @@ int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, u
 }
 
 int launch_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_work,
-                        hipStream_t st) {
+                        uint64_t* d_out, hipStream_t st) {
     hipLaunchKernelGGL(k_verify_slow, dim3(PFC_VERIFY_BLOCKS), dim3(256), 0, st, kind, N, d_work);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "k_verify_slow launch");
-    hipLaunchKernelGGL(k_verify_finish, dim3(1), dim3(kVerifyLanes), 0, st, *t, d_work);
+    hipLaunchKernelGGL(k_verify_finish, dim3(1), dim3(kVerifyLanes), 0, st, *t, (const uint64_t*)d_work, d_out);
     e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_verify_finish launch");
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array at its own address).
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch[kMaxScratchDevices];
+
+static int scratch_address(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxScratchDevices) return set_error(PFC_ERR_INVALID, "device ordinal beyond 63");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return PFC_OK;
 }
 
 int probe_copy(uint64_t bytes, double* ms_out) {
@@ -1267,7 +1300,25 @@ int pfc_stage_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, 
         return pfc::set_error(PFC_ERR_INVALID, "verify_table: table pointers missing");
     if (t->digit_bits < 1 || t->digit_bits > 8)
         return pfc::set_error(PFC_ERR_INVALID, "verify_table: digit_bits must be 1..8");
-    return pfc::launch_verify_table(kind, N, t, d_work, (hipStream_t)stream);
+    return pfc::launch_verify_table(kind, N, t, d_work, d_work, (hipStream_t)stream);
+}
+
+int pfc_stage_verify_table_noalloc(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_out,
+                                   void* stream) {
+    pfc::g_err[0] = 0;
+    if (!t || !d_out) return pfc::set_error(PFC_ERR_INVALID, "verify_table_noalloc: null pointer");
+    if (kind > 2) return pfc::set_error(PFC_ERR_INVALID, "verify_table_noalloc: kind must be 0, 1 or 2");
+    if (N > 0x7FFFFFFFu) return pfc::set_error(PFC_ERR_INVALID, "verify_table_noalloc: N >= 2^31");
+    if (t->npos && (!t->chunks || !t->groups))
+        return pfc::set_error(PFC_ERR_INVALID, "verify_table_noalloc: table pointers missing");
+    if (t->digit_bits < 1 || t->digit_bits > 8)
+        return pfc::set_error(PFC_ERR_INVALID, "verify_table_noalloc: digit_bits must be 1..8");
+    void* work = nullptr;
+    const int rc = pfc::scratch_address(&work);
+    if (rc) return rc;
+    // Every block partial is written by k_verify_slow before k_verify_finish
+    // reads it, so nothing in the scratch needs clearing between calls.
+    return pfc::launch_verify_table(kind, N, t, (uint64_t*)work, d_out, (hipStream_t)stream);
 }
 
 }  // extern "C"
