@@ -286,6 +286,51 @@ def serve_prefix(dev, njobs, mix, round_updates, progress):
             "round_policy": f"about {round_updates:.3g} p*n updates per round"}
 
 
+def kernel_rates(dev, reps=3):
+    """Standalone rates of the kernels around the walk: candidate generation
+    (k_generate through pfc_stage_generate, the whole p-window of the
+    headline job and of the primorial job near 2^62, best of `reps`), and
+    start-residue set-up (k_setup) of 999999! per candidate on one
+    p-segment from p = 1e15."""
+    import ctypes as C
+    import pfcsieve_amd as P
+    import pfcsieve_amd._native as N
+    from pfcsieve_amd.sieve import DeviceBuffer
+    out = {}
+    for name, lo, hi in (("generate_p1e12_2e9", 10**12, 10**12 + 2 * 10**9),
+                         ("generate_p2e62_4e9", 2**62 - 4 * 10**9, 2**62)):
+        span = hi - lo
+        cap = int(2.2 * span / math.log(span)) + 4096
+        d_p = DeviceBuffer(dev, 8 * cap)
+        ctrl = DeviceBuffer(dev, N.CTRL_BYTES)
+        best = None
+        for _ in range(reps):
+            ctrl.zero()
+            N.check(N.lib().pfc_device_synchronize(dev.handle))
+            t = time.perf_counter()
+            N.check(N.lib().pfc_stage_generate(lo, hi, d_p.ptr, cap, ctrl.ptr, None))
+            N.check(N.lib().pfc_device_synchronize(dev.handle))
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        n = int.from_bytes(ctrl.download(4), "little")
+        out[name] = {"window": f"p in [{lo}, {hi})", "candidates": n, "s": best, "candidates_per_s": n / best,
+                     "integers_per_s": span / best}
+        d_p.free()
+        ctrl.free()
+    products = P.setup_table_cost(0, 10**6 - 1)
+    j = P.Job(dev, 1, 10**15, 10**15 + 2 * 10**9, 10**6, 10**6 + 256)
+    j.run(1)
+    s = j.stats()
+    j.destroy()
+    setup_s = (s.device_ms - s.walk_ms) * 1e-3
+    out["setup_nmin_1e6"] = {"workload": "-! n [1e6, 1e6+256) p from 1e15, one p-segment: start residue "
+                                         "999999! mod p per candidate (generate and check included, ~1%)",
+                             "candidates": s.primecount, "modular_products_per_candidate": products,
+                             "s": setup_s, "candidates_per_s": s.primecount / setup_s,
+                             "products_per_s": s.primecount * products / setup_s}
+    return out
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -343,6 +388,7 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=200, help="jobs of the seeded served stream (0 = skip)")
     ap.add_argument("--serve-round-updates", type=float, default=4e11)
     ap.add_argument("--no-headline", action="store_true")
+    ap.add_argument("--kernel-rates", action="store_true", help="also time k_generate and k_setup standalone")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-width", type=int, default=1_000_000)
     ap.add_argument("--progress", action="store_true", help="progress lines on stderr")
@@ -360,13 +406,14 @@ def main():
     mix = measure_mix(dev, K, W, args.progress, args.mix_units) if args.mix_units else None
     line = {
         "metric": "p·n residue updates/sec on 1 MI355X, batched vs plain; % INT32 VALU peak",
-        "value": mix["batched_updates_per_s"] if mix else head["updates_per_s"], "unit": "updates/s",
-        "n_gpus": 1, "steps": mix["rounds"] if mix else K, "warmup": W,
-        "ms_per_step": (mix["batched_s"] / mix["rounds"] if mix else head["wall_s"] / K) * 1e3,
+        "value": mix["batched_updates_per_s"] if mix else head["updates_per_s"] if head else None,
+        "unit": "updates/s", "n_gpus": 1, "steps": mix["rounds"] if mix else K, "warmup": W,
+        "ms_per_step": (mix["batched_s"] / mix["rounds"] * 1e3 if mix else head["wall_s"] / K * 1e3 if head
+                        else None),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u64 (exact modular integer)", "data": "synthetic (seeded job mix; device-generated primes)",
         "config": {"workload": (f"batched uneven mix: {mix['units']} seeded units (splitmix64 seed 12345)"
-                                if mix else "headline -! -c job"),
+                                if mix else "headline -! -c job" if head else "kernel rates only"),
                    "form": "batched: all units through one batcher, persistent k_batch launches; a step is one "
                            "batcher round, the K rounds sized by work; plain beside it under 'mix'",
                    "mix": mix, "headline": head, "tuning": dev.tuning()},
@@ -376,6 +423,8 @@ def main():
     if mix and args.serve_jobs:
         line["served"] = serve_prefix(dev, args.serve_jobs, mix, int(args.serve_round_updates), args.progress)
     line["config"]["small_job"] = small_job(dev)
+    if args.kernel_rates:
+        line["kernel_rates"] = kernel_rates(dev)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_width)
         if head:

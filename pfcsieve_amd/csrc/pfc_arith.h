@@ -261,6 +261,51 @@ PFC_HD bool strong_prp2(uint64_t N) {
     return false;
 }
 
+// Base-2 strong probable-prime test of K odd values N >= 3 in lockstep, with
+// no data-dependent branch: for M = N - 1 and t = ctz(M), a = 2^(M >> i) mod N
+// is built over the bits of M from the top (a <- a^2, then a <- 2a where bit
+// i of M is set; while i is above M's leading bit, a stays 1, since 1^2 = 1),
+// so the trip count is the same for every value of bit length up to `bits`.
+// At i = t, a = 2^e with e the odd part of M (the strong test's x0, which
+// passes as +-1); at each i from t - 1 down to 1, a = x0^(2^(t-i)), which
+// passes as -1.  `tmax` >= every t: the rounds above it skip the checks.  For
+// N = 2^t + 1 (e = 1) the test starts from 2^1, as strong_prp2 does.  On the
+// device `bits` and `tmax` are wavefront-uniform, so K candidates per lane run
+// as K interleaved squaring chains.
+template <int K>
+PFC_HD void strong_prp2_lockstep(const uint64_t (&N)[K], uint32_t bits, uint32_t tmax, bool (&prp)[K]) {
+    Modulus m[K];
+    uint64_t d[K], one[K], nmo[K], M[K], a[K];
+    uint32_t t[K];
+    for (int k = 0; k < K; ++k) {
+        m[k] = make_modulus(N[k]);
+        d[k] = mod_d(m[k]);
+        one[k] = 1ull << m[k].s;
+        nmo[k] = d[k] - one[k];
+        M[k] = N[k] - 1;
+        t[k] = (uint32_t)__builtin_ctzll(M[k]);
+        a[k] = one[k];
+        prp[k] = false;
+    }
+    int i = (int)bits - 1;
+    for (; i > (int)tmax; --i) {
+        for (int k = 0; k < K; ++k) {
+            a[k] = mulmod64(a[k], a[k] >> m[k].s, m[k]);
+            const uint64_t a2 = addmod(a[k], a[k], m[k]);
+            a[k] = ((M[k] >> i) & 1) ? a2 : a[k];
+        }
+    }
+    for (; i >= 1; --i) {
+        for (int k = 0; k < K; ++k) {
+            a[k] = mulmod64(a[k], a[k] >> m[k].s, m[k]);
+            const uint64_t a2 = addmod(a[k], a[k], m[k]);
+            a[k] = ((M[k] >> i) & 1) ? a2 : a[k];
+            const uint32_t ui = (uint32_t)i;
+            prp[k] |= (ui <= t[k] && a[k] == nmo[k]) || (ui == t[k] && a[k] == one[k]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- 32-bit primality
 
 // Montgomery product a*b*2^-32 mod n for odd n < 2^32 and a, b < n;

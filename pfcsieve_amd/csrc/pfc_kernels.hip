@@ -44,6 +44,7 @@ namespace {
 constexpr int kGenBlock = 256;
 constexpr int kGenSpan = 60;                      // integers per window (one 64-bit mask)
 constexpr int kGenLdsPerWindow = kGenBlock * 16;  // at most 16 survivors per window
+constexpr int kGenPrpK = 2;                       // PRP tests per lane per trip, interleaved
 
 // Appends values to out[] with one atomic per wavefront.
 __device__ __forceinline__ void wave_append(bool keep, uint64_t value, uint64_t* out, uint32_t cap,
@@ -63,51 +64,129 @@ __device__ __forceinline__ void wave_append(bool keep, uint64_t value, uint64_t*
     }
 }
 
+// Inclusive prefix sum of v over the wavefront's lanes.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o);
+        if (lane >= (uint32_t)o) v += u;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 // The LDS segment of one workgroup is W*15360 integers: block b covers
 // [base30 + b*W*15360, +W*15360), and window w of thread t the 60 integers
 // from base30 + b*W*15360 + 60*(w*256 + t); only integers in [lo, hi) are
 // kept.  Survivors of the wheel and the 7..113 sieve are compacted in LDS,
 // then PRP-tested by the whole block (the reference's shape,
-// getsegprimes.cl:193-223, with the LDS list bounded by construction: at most
-// 16 survivors per window).  W (1, 2, 4) is the tunable LDS segment size.
+// getsegprimes.cl:193-223).  The LDS list holds 16-bit offsets from the
+// block's base (below W*15360 <= 61440), at most 16 per window, so it cannot
+// overflow and takes 8 KB per 15360 integers: eight 256-lane workgroups fit a
+// CU's LDS, the full wave occupancy.  Each lane writes its survivors at its
+// offset in a workgroup-wide prefix sum of the survivor counts (no
+// per-survivor atomic).  The PRP phase takes kGenPrpK survivors per lane per
+// trip through pfc::strong_prp2_lockstep, whose trip count is uniform (the
+// bit length of the block's last integer and the wavefront's largest t), so
+// the lanes and their kGenPrpK squaring chains run without divergence.  The
+// primes found are compacted into the front of the same LDS list (a trip's
+// writes land below the next trip's entries, and every lane has read its
+// entries before the trip's barrier), then appended to the output with ONE
+// global atomic per workgroup: a counter shared by the whole grid, updated
+// once per wavefront, had set the kernel's pace.  W (1, 2, 4) is the tunable
+// LDS segment size.
 template <int W>
 __device__ __forceinline__ void generate_block(uint64_t base30, uint64_t lo, uint64_t hi,
                                                uint64_t* out, uint32_t cap, pfc_ctrl* ctrl) {
-    __shared__ uint64_t s_list[kGenLdsPerWindow * W];
+    __shared__ uint16_t s_list[kGenLdsPerWindow * W];
     __shared__ uint32_t s_mod[27];
-    __shared__ uint32_t s_count;
+    __shared__ uint32_t s_wsum[kGenBlock / 64];
+    __shared__ uint32_t s_kept, s_gbase;
     const uint32_t tid = threadIdx.x;
     const uint64_t block_base = base30 + (uint64_t)blockIdx.x * (kGenBlock * kGenSpan * W);
     if (tid < 27) s_mod[tid] = (uint32_t)(block_base % pfc::kSievePrimes[tid]);
-    if (tid == 0) s_count = 0;
+    if (tid == 0) s_kept = 0;
     __syncthreads();
 
+    uint64_t cand[W];
+    uint32_t cnt = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         const uint32_t off = (uint32_t)(w * kGenBlock + tid) * kGenSpan;
-        const uint64_t start = block_base + off;
         uint32_t rmod[27];
 #pragma unroll
         for (int i = 0; i < 27; ++i) rmod[i] = (s_mod[i] + off) % pfc::kSievePrimes[i];
-        uint64_t cand = pfc::window_survivors(start, lo, hi, rmod);
-        while (cand) {
-            const uint32_t j = __ffsll((unsigned long long)cand) - 1;
-            cand &= cand - 1;
-            s_list[atomicAdd(&s_count, 1u)] = start + j;
+        cand[w] = pfc::window_survivors(block_base + off, lo, hi, rmod);
+        cnt += (uint32_t)__popcll(cand[w]);
+    }
+    // Workgroup prefix sum of the counts: lanes within a wavefront, then the
+    // wavefront totals through LDS.
+    const uint32_t incl = wave_inclusive_sum(cnt);
+    const uint32_t wave = tid >> 6;
+    if ((tid & 63u) == 63u) s_wsum[wave] = incl;
+    __syncthreads();
+    uint32_t pos = incl - cnt, n = 0;
+#pragma unroll
+    for (uint32_t v = 0; v < kGenBlock / 64; ++v) {
+        const uint32_t ws = s_wsum[v];
+        if (v < wave) pos += ws;
+        n += ws;
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const uint32_t off = (uint32_t)(w * kGenBlock + tid) * kGenSpan;
+        for (uint64_t c = cand[w]; c; c &= c - 1) s_list[pos++] = (uint16_t)(off + (uint32_t)__builtin_ctzll(c));
+    }
+    __syncthreads();
+
+    // Every integer of the block has at most the bit length of its last one.
+    const uint64_t top = block_base + (uint64_t)(kGenBlock * kGenSpan * W) - 1;
+    const uint32_t bits = 64u - (uint32_t)__builtin_clzll(top < block_base ? ~0ull : top);
+    // Every lane of the workgroup takes every trip (n is uniform): the
+    // barrier in the loop is reached by all.
+    for (uint32_t i0 = 0; i0 < n; i0 += kGenPrpK * kGenBlock) {
+        uint64_t c[kGenPrpK];
+        bool valid[kGenPrpK];
+        uint32_t t = 0;
+#pragma unroll
+        for (int k = 0; k < kGenPrpK; ++k) {
+            const uint32_t i = i0 + k * kGenBlock + tid;
+            valid[k] = i < n;
+            // an idle chain tests a copy of a valid value (or 3): any odd N >= 3 is a valid input
+            c[k] = valid[k] ? block_base + s_list[i] : (k > 0 && valid[0] ? c[0] : 3ull);
+            t = max(t, (uint32_t)__builtin_ctzll(c[k] - 1));
+        }
+        __syncthreads();                  // this trip's entries are read before any is overwritten
+        bool prp[kGenPrpK];
+        pfc::strong_prp2_lockstep<kGenPrpK>(c, max(bits, 2u), wave_max(t), prp);
+#pragma unroll
+        for (int k = 0; k < kGenPrpK; ++k) {
+            const bool keep = valid[k] && prp[k];
+            const uint64_t ballot = __ballot(keep);
+            if (ballot == 0) continue;
+            const uint32_t lane = __lane_id();
+            const int leader = __ffsll((unsigned long long)ballot) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&s_kept, (uint32_t)__popcll(ballot));
+            base = __shfl(base, leader);
+            if (keep) s_list[base + (uint32_t)__popcll(ballot & ((1ull << lane) - 1))] = (uint16_t)(c[k] - block_base);
         }
     }
     __syncthreads();
-    const uint32_t n = s_count;
-    // Whole wavefronts iterate together so the append can use ballots.
-    for (uint32_t i0 = 0; i0 < n; i0 += kGenBlock) {
-        const uint32_t i = i0 + tid;
-        bool keep = false;
-        uint64_t c = 0;
-        if (i < n) {
-            c = s_list[i];
-            keep = pfc::strong_prp2(c);
-        }
-        wave_append(keep, c, out, cap, ctrl);
+    const uint32_t nk = s_kept;
+    if (tid == 0) s_gbase = nk ? atomicAdd(&ctrl->ncand, nk) : 0u;
+    __syncthreads();
+    const uint32_t g = s_gbase;
+    for (uint32_t i = tid; i < nk; i += kGenBlock) {
+        const uint32_t idx = g + i;
+        if (idx < cap) out[idx] = block_base + s_list[i];
+        else atomicOr(&ctrl->flags, 1u);
     }
 }
 

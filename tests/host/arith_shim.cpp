@@ -27,6 +27,20 @@ uint64_t t_mont_one(uint64_t p) { return pfc::mont_one(pfc::make_modulus(p)); }
 int t_modulus_ok(uint64_t p) { return pfc::modulus_ok(p, pfc::make_modulus(p)); }
 int t_strong_prp2(uint64_t n) { return pfc::strong_prp2(n); }
 int t_is_prime_u32(uint32_t n) { return pfc::is_prime_u32(n); }
+// The lockstep test on two values with shared bit and t bounds (the largest
+// of the pair, as a wavefront uses them); out[0..1] = verdicts.
+void t_strong_prp2_pair(uint64_t n0, uint64_t n1, int out[2]) {
+    const uint64_t N[2] = {n0, n1};
+    uint32_t bits = 0, tmax = 0;
+    for (int k = 0; k < 2; ++k) {
+        bits = bits > 64u - (uint32_t)__builtin_clzll(N[k] - 1) ? bits : 64u - (uint32_t)__builtin_clzll(N[k] - 1);
+        tmax = tmax > (uint32_t)__builtin_ctzll(N[k] - 1) ? tmax : (uint32_t)__builtin_ctzll(N[k] - 1);
+    }
+    bool prp[2];
+    pfc::strong_prp2_lockstep<2>(N, bits, tmax, prp);
+    out[0] = prp[0];
+    out[1] = prp[1];
+}
 uint64_t t_start_residue(const uint64_t* chunks, const uint32_t* groups, uint32_t npos, uint32_t k, uint64_t p) {
     return pfc::start_residue(chunks, groups, npos, k, pfc::make_modulus(p));
 }

@@ -192,6 +192,7 @@ def A():
         getattr(L, n).restype = u64
     L.t_modulus_ok.argtypes = [u64]
     L.t_strong_prp2.argtypes = [u64]
+    L.t_strong_prp2_pair.argtypes = [u64, u64, C.POINTER(C.c_int * 2)]
     L.t_is_prime_u32.argtypes = [u32]
     L.t_generate.argtypes = [u64, u64, C.POINTER(u64), C.c_size_t]
     L.t_generate.restype = C.c_size_t
@@ -331,6 +332,25 @@ def test_strong_prp2_device_arith(A, orc):
     xs += [(1 << t) + 1 for t in (8, 16, 32, 40, 48, 56)] + [3215031751, 2152302898747, 3474749660383]
     for x in xs:
         assert bool(A.t_strong_prp2(x)) == orc.strong_prp2(x), x
+
+
+def test_strong_prp2_lockstep_pairs(A, orc):
+    """The generator's lockstep test, two values with shared loop bounds (the
+    larger bit length and t of the pair): each verdict equals the oracle's
+    strong PRP, across bit-length boundaries, large t (N = 2^t * odd + 1),
+    Fermat-form N = 2^t + 1, known base-2 strong pseudoprimes and the top of
+    the 64-bit range."""
+    rng = random.Random(6)
+    xs = list(range(3, 2000, 2)) + [rng.randrange(3, R) | 1 for _ in range(2000)]
+    xs += [(1 << t) + 1 for t in (8, 16, 32, 40, 48, 56, 63)] + [3215031751, 2152302898747, 3474749660383]
+    xs += [(rng.randrange(1, 1 << 20) | 1) << t | 1 for t in (20, 30, 40) for _ in range(20)]
+    xs += [R - 59, R - 1, (1 << 63) + 1, (1 << 62) - 57]
+    xs = [x for x in xs if 3 <= x < R]
+    out = (C.c_int * 2)()
+    for k in range(len(xs)):
+        a, b = xs[k], xs[rng.randrange(len(xs))]
+        A.t_strong_prp2_pair(a, b, out)
+        assert (bool(out[0]), bool(out[1])) == (orc.strong_prp2(a), orc.strong_prp2(b)), (a, b)
 
 
 def test_is_prime_u32(A, orc):
