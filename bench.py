@@ -11,8 +11,10 @@ Workloads (synthetic: every candidate prime is generated on the device):
   (splitmix64 seed 12345; modes F/P/C with equal odds, n-width log-U[1e3,1e6]
   from 101+U[0,1e6], p-width log-U[1e6,1e9] from log-U[1e10,2^63]), every
   unit one p-segment.  Batched: all units in one batcher, timed from job
-  creation to job destruction; a step is one batcher round (one persistent
-  k_batch launch), and the K rounds are sized by work (round r walks about
+  creation to job destruction; a step is one batcher round (its device
+  phase -- work-item upload, queue clear, one persistent k_batch launch,
+  control-block readback -- replayed from one captured HIP graph), and the K
+  rounds are sized by work (round r walks about
   the remaining updates / (K - r); the last walks everything left), so every
   step does work and the K steps cover the mix.  Plain (``mix.plain``): the
   same units one job after another, one range per launch sequence.
@@ -229,6 +231,8 @@ def measure_mix(dev, K, W, progress, nunits=256):
             j.destroy()
             _progress(progress, f"plain: unit {k + 1}/{len(units)} done, {time.perf_counter() - t:.1f} s")
         t_plain = time.perf_counter() - t
+        # batched: every round's device phase replayed from one captured HIP graph
+        dev.set_tuning(seg_candidates=1 << 26, use_graph=1)
         t_batch, rounds, batched, stats = batched_rounds(dev, units, K, progress)
     finally:
         dev.set_tuning()
@@ -414,16 +418,30 @@ def main():
         "dtype": "u64 (exact modular integer)", "data": "synthetic (seeded job mix; device-generated primes)",
         "config": {"workload": (f"batched uneven mix: {mix['units']} seeded units (splitmix64 seed 12345)"
                                 if mix else "headline -! -c job" if head else "kernel rates only"),
-                   "form": "batched: all units through one batcher, persistent k_batch launches; a step is one "
-                           "batcher round, the K rounds sized by work; plain beside it under 'mix'",
+                   "form": "batched: all units through one batcher, one persistent k_batch launch per round "
+                           "replayed from a captured HIP graph; a step is one batcher round, the K rounds sized "
+                           "by work; plain (one job after another, eager) beside it under 'mix'",
                    "mix": mix, "headline": head, "tuning": dev.tuning()},
     }
     if head:
         line["roofline"] = roofline(head, rates)
+    if args.kernel_rates or (head and mix):
+        line["kernel_rates"] = kernel_rates(dev)
+    if head and mix:
+        # The mix's device work at the kernels' own measured rates: walk updates
+        # at the headline walk rate, start-residue products at the set-up rate,
+        # candidates at the generator's rate on the 1e12 window.
+        kr = line["kernel_rates"]
+        bound = (mix["walk_updates"] / head["walk_updates_per_s"]
+                 + mix["setup_products"] / kr["setup_nmin_1e6"]["products_per_s"]
+                 + mix["candidates"] / kr["generate_p1e12_2e9"]["candidates_per_s"])
+        mix["device_work_bound_s"] = bound
+        mix["batched_frac_of_bound"] = bound / mix["batched_s"]
+        mix["plain_frac_of_bound"] = bound / mix["plain_s"]
     if mix and args.serve_jobs:
         line["served"] = serve_prefix(dev, args.serve_jobs, mix, int(args.serve_round_updates), args.progress)
     line["config"]["small_job"] = small_job(dev)
-    if args.kernel_rates:
+    if args.kernel_rates and "kernel_rates" not in line:
         line["kernel_rates"] = kernel_rates(dev)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_width)

@@ -74,7 +74,10 @@ typedef struct pfc_tuning {
     uint32_t walk_chunk;       /* multiplier steps per walk launch (default 2^18) */
     uint32_t seg_candidates;   /* target candidates per p-segment (default 2^24) */
     uint32_t use_graph;        /* 1: capture a job's segment pipeline (clear, generate, setup,
-                                  walk chunks, check) once as a HIP graph and replay it per segment */
+                                  walk chunks, check) once as a HIP graph and replay it per segment;
+                                  and a batcher round's device phase (work-item upload, queue clear,
+                                  the persistent set-up + walk + fold launch, control-block readback)
+                                  once per batcher, replayed every round */
     uint32_t skip_table_check; /* 1: do not self-check the start tables at job creation
                                   (pfc_stage_verify_table); 0 (default): check them */
     uint32_t lds_segment;      /* integers sieved per generator workgroup and staged in LDS:

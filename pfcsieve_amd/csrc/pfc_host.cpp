@@ -1249,10 +1249,24 @@ struct pfc_batcher {
     size_t d_jobs_cap = 0;
     pfc_batch_range* d_ranges = nullptr;
     size_t d_ranges_cap = 0;
-    uint32_t* d_head = nullptr;
+    uint32_t* d_words = nullptr;      // {queue head, range count, slot count}
     pfc_ctrl* h_ctrl = nullptr;       // pinned, one per slot
     size_t h_ctrl_cap = 0;
     uint64_t rounds = 0;
+    // Captured device phase of a round (tuning.use_graph): uploads from the
+    // pinned staging below, queue-head clear, k_batch, control-block gather
+    // and one readback, replayed every round; captured again only when a
+    // buffer capacity or the launch shape changes.
+    pfc_batch_job* h_jobs = nullptr;  // pinned staging, d_jobs_cap entries
+    pfc_batch_range* h_ranges = nullptr;  // pinned staging, d_ranges_cap entries
+    uint32_t* h_words = nullptr;      // pinned: {range count, slot count}
+    pfc_ctrl* d_ctrl_all = nullptr;   // gathered control blocks, d_jobs_cap entries
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    size_t graph_jobs_cap = 0, graph_ranges_cap = 0;
+    const pfc_ctrl* graph_h_ctrl = nullptr;   // the readback target the graph was captured with
+    uint32_t graph_k = 0;
+    uint64_t graph_replays = 0;
 };
 
 namespace {
@@ -1290,6 +1304,22 @@ int ensure_device(T** p, size_t* cap, size_t need) {
     if (e != hipSuccess) {
         *cap = 0;
         return hip_fail(e, "batcher allocation");
+    }
+    *cap = c;
+    return PFC_OK;
+}
+
+// A pinned host array of at least `need` entries (contents not kept).
+template <class T>
+int ensure_pinned(T** p, size_t* cap, size_t need) {
+    if (need <= *cap) return PFC_OK;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    const size_t c = std::max<size_t>(need, 2 * *cap);
+    hipError_t e = hipHostMalloc((void**)p, c * sizeof(T), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *cap = 0;
+        return hip_fail(e, "batcher pinned allocation");
     }
     *cap = c;
     return PFC_OK;
@@ -1337,7 +1367,8 @@ extern "C" int pfc_batcher_create(pfc_device* dev, pfc_batcher** out) {
     pfc_batcher* b = new pfc_batcher();
     b->dev = dev;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->d_head, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_words, 4 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_words, 4 * sizeof(uint32_t), hipHostMallocDefault);
     if (e != hipSuccess) {
         int rc = hip_fail(e, "batcher create");
         if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1364,6 +1395,89 @@ extern "C" int pfc_batcher_pending(const pfc_batcher* b, size_t* n) {
 
 // Steps per segment of a work-budgeted round, at least this many.
 constexpr uint32_t kMinRoundSteps = 256;
+
+// The device phase of a round from one captured graph: the job views and
+// ranges are staged in pinned host memory, which the graph's copy nodes read
+// at each replay; the queue head is cleared by a kernel (no memset node);
+// k_batch reads the range count from device memory and runs on a fixed grid
+// (wavefronts past the queue's end return at once); every slot's control
+// block is gathered into h_ctrl with one readback.  Captured again only when
+// a buffer grows or the launch shape changes.
+static int batcher_graph_round(pfc_batcher* b, const std::vector<pfc_batch_job>& jobs,
+                               const std::vector<pfc_batch_range>& flat, uint32_t K) {
+    hipStream_t st = b->stream;
+    const size_t ns = jobs.size();
+    size_t hj_cap = b->graph_jobs_cap, hr_cap = b->graph_ranges_cap;
+    int rc = PFC_OK;
+    const bool recapture = !b->gexec || b->graph_jobs_cap != b->d_jobs_cap ||
+                           b->graph_ranges_cap != b->d_ranges_cap || b->graph_k != K ||
+                           b->h_ctrl_cap < b->d_jobs_cap || b->graph_h_ctrl != b->h_ctrl;
+    if (recapture) {
+        if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+        // staging and gather buffers at the device buffers' capacities
+        if (b->h_jobs) (void)hipHostFree(b->h_jobs);
+        if (b->h_ranges) (void)hipHostFree(b->h_ranges);
+        if (b->d_ctrl_all) (void)hipFree(b->d_ctrl_all);
+        b->h_jobs = nullptr;
+        b->h_ranges = nullptr;
+        b->d_ctrl_all = nullptr;
+        hj_cap = hr_cap = 0;
+        size_t cc = 0;
+        rc = ensure_pinned(&b->h_jobs, &hj_cap, b->d_jobs_cap);
+        if (!rc) rc = ensure_pinned(&b->h_ranges, &hr_cap, b->d_ranges_cap);
+        if (!rc) rc = ensure_device(&b->d_ctrl_all, &cc, b->d_jobs_cap);
+        if (!rc && b->h_ctrl_cap < b->d_jobs_cap) {
+            if (b->h_ctrl) (void)hipHostFree(b->h_ctrl);
+            b->h_ctrl = nullptr;
+            b->h_ctrl_cap = 0;
+            rc = ensure_pinned(&b->h_ctrl, &b->h_ctrl_cap, b->d_jobs_cap);
+        }
+        if (rc) return rc;
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_fail(e, "batcher hipStreamBeginCapture");
+        e = hipMemcpyAsync(b->d_jobs, b->h_jobs, b->d_jobs_cap * sizeof(pfc_batch_job), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->d_ranges, b->h_ranges, b->d_ranges_cap * sizeof(pfc_batch_range),
+                               hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->d_words + 1, b->h_words, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) rc = hip_fail(e, "batcher graph upload");
+        if (!rc) rc = pfc::launch_zero_u32(b->d_words, st);
+        if (!rc)
+            rc = pfc::launch_batch(b->d_jobs, b->d_ranges, b->d_words + 1, b->d_words, K,
+                                   (uint32_t)b->dev->prop.multiProcessorCount * 32, st);
+        if (!rc) rc = pfc::launch_gather_ctrl(b->d_jobs, b->d_words + 2, b->d_ctrl_all, (uint32_t)b->d_jobs_cap, st);
+        if (!rc) {
+            e = hipMemcpyAsync(b->h_ctrl, b->d_ctrl_all, b->d_jobs_cap * sizeof(pfc_ctrl), hipMemcpyDeviceToHost, st);
+            if (e != hipSuccess) rc = hip_fail(e, "batcher graph readback");
+        }
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "batcher hipStreamEndCapture");
+        b->graph = g;
+        e = hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) return hip_fail(e, "batcher hipGraphInstantiate");
+        b->graph_jobs_cap = b->d_jobs_cap;
+        b->graph_ranges_cap = b->d_ranges_cap;
+        b->graph_h_ctrl = b->h_ctrl;
+        b->graph_k = K;
+    }
+    memcpy(b->h_jobs, jobs.data(), ns * sizeof(pfc_batch_job));
+    memcpy(b->h_ranges, flat.data(), flat.size() * sizeof(pfc_batch_range));
+    b->h_words[0] = (uint32_t)flat.size();
+    b->h_words[1] = (uint32_t)ns;
+    hipError_t e = hipGraphLaunch(b->gexec, st);
+    if (e != hipSuccess) return hip_fail(e, "batcher hipGraphLaunch");
+    b->graph_replays += 1;
+    return PFC_OK;
+}
 
 // One round; max_updates != 0 replaces max_steps by the step count that gives
 // about max_updates p*n updates over all walking segments (F|C counted twice).
@@ -1464,6 +1578,7 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
         if (j1 == j->jend) finishing[i] = 1;
     }
     std::stable_sort(ranges.begin(), ranges.end(), [](const Tmp& a, const Tmp& c) { return a.cost > c.cost; });
+    bool gathered = false;                 // every slot's control block is in h_ctrl after the sync
     if (!ranges.empty()) {
         rc = ensure_device(&b->d_jobs, &b->d_jobs_cap, ns);
         if (!rc) rc = ensure_device(&b->d_ranges, &b->d_ranges_cap, ranges.size());
@@ -1475,17 +1590,31 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
             flat[k] = ranges[k].r;
             flat[k].end = end;
         }
-        hipError_t e = hipMemcpyAsync(b->d_jobs, jobs.data(), ns * sizeof(pfc_batch_job), hipMemcpyHostToDevice, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(b->d_ranges, flat.data(), flat.size() * sizeof(pfc_batch_range), hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) e = hipMemsetAsync(b->d_head, 0, sizeof(uint32_t), st);
-        if (e != hipSuccess) return hip_fail(e, "batcher upload");
-        const uint32_t nblocks = (uint32_t)std::min<uint64_t>(nitems, (uint64_t)b->dev->prop.multiProcessorCount * 32);
-        rc = pfc::launch_batch(b->d_jobs, b->d_ranges, (uint32_t)flat.size(), b->d_head, K, nblocks, st);
-        if (rc) return rc;
+        bool use_graph;
+        {
+            std::lock_guard<std::mutex> g(b->dev->mu);
+            use_graph = b->dev->tuning.use_graph != 0;
+        }
+        if (use_graph) {
+            rc = batcher_graph_round(b, jobs, flat, K);
+            if (rc) return rc;
+            gathered = true;
+        } else {
+            b->h_words[0] = (uint32_t)flat.size();
+            hipError_t e = hipMemcpyAsync(b->d_jobs, jobs.data(), ns * sizeof(pfc_batch_job), hipMemcpyHostToDevice, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(b->d_ranges, flat.data(), flat.size() * sizeof(pfc_batch_range), hipMemcpyHostToDevice,
+                                   st);
+            if (e == hipSuccess) e = hipMemcpyAsync(b->d_words + 1, b->h_words, sizeof(uint32_t), hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipMemsetAsync(b->d_words, 0, sizeof(uint32_t), st);
+            if (e != hipSuccess) return hip_fail(e, "batcher upload");
+            const uint32_t nblocks = (uint32_t)std::min<uint64_t>(nitems, (uint64_t)b->dev->prop.multiProcessorCount * 32);
+            rc = pfc::launch_batch(b->d_jobs, b->d_ranges, b->d_words + 1, b->d_words, K, nblocks, st);
+            if (rc) return rc;
+        }
     }
     // 3. drain finished segments
-    for (size_t i = 0; i < ns; ++i) {
+    for (size_t i = 0; i < ns && !gathered; ++i) {
         if (!finishing[i]) continue;
         hipError_t e = hipMemcpyAsync(&b->h_ctrl[i], b->slots[i].job->d_ctrl, sizeof(pfc_ctrl),
                                       hipMemcpyDeviceToHost, st);
@@ -1534,8 +1663,14 @@ extern "C" int pfc_batcher_destroy(pfc_batcher* b) {
     (void)hipStreamSynchronize(b->stream);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
     if (b->d_ranges) (void)hipFree(b->d_ranges);
-    if (b->d_head) (void)hipFree(b->d_head);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->d_words) (void)hipFree(b->d_words);
+    if (b->d_ctrl_all) (void)hipFree(b->d_ctrl_all);
     if (b->h_ctrl) (void)hipHostFree(b->h_ctrl);
+    if (b->h_jobs) (void)hipHostFree(b->h_jobs);
+    if (b->h_ranges) (void)hipHostFree(b->h_ranges);
+    if (b->h_words) (void)hipHostFree(b->h_words);
     (void)hipStreamDestroy(b->stream);
     delete b;
     return PFC_OK;

@@ -67,8 +67,13 @@ int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list
 int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint32_t* d_v,
                  const uint64_t* d_one, const uint64_t* d_ra, const uint64_t* d_rb,
                  pfc_ctrl* d_ctrl, uint32_t cap, hipStream_t st);
-int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, uint32_t nranges,
+// d_nranges: device word holding the range count (read by the kernel).
+int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, const uint32_t* d_nranges,
                  uint32_t* d_head, uint32_t ppl, uint32_t nblocks, hipStream_t st);
+int launch_zero_u32(uint32_t* d_p, hipStream_t st);
+// d_out[i] = *d_jobs[i].ctrl for i < *d_nslots (cap: the most slots).
+int launch_gather_ctrl(const pfc_batch_job* d_jobs, const uint32_t* d_nslots, pfc_ctrl* d_out, uint32_t cap,
+                       hipStream_t st);
 // d_work: PFC_VERIFY_WORDS words (block partials in [2, PFC_VERIFY_WORDS));
 // the two results go to d_out[0], d_out[1] (d_out may be d_work).
 int launch_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_work,

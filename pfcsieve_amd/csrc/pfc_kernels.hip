@@ -1001,11 +1001,16 @@ __device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_bat
 }
 
 // The queue is a counter over the launch's items; an item's job segment is
-// found by a wavefront-uniform binary search over the ranges' ends.
+// found by a wavefront-uniform binary search over the ranges' ends.  The
+// range count is read from device memory, so that one captured launch
+// serves every round of a batcher (the host writes the count before each
+// replay); a launch with no ranges returns at once.
 template <int K>
 __global__ __launch_bounds__(64) void k_batch(const pfc_batch_job* __restrict__ jobs,
                                               const pfc_batch_range* __restrict__ ranges,
-                                              uint32_t nranges, uint32_t* head) {
+                                              const uint32_t* __restrict__ nranges_p, uint32_t* head) {
+    const uint32_t nranges = *nranges_p;
+    if (nranges == 0) return;
     const uint32_t nitems = ranges[nranges - 1].end;
     for (;;) {
         uint32_t i = 0;
@@ -1030,6 +1035,21 @@ __global__ __launch_bounds__(64) void k_batch(const pfc_batch_job* __restrict__ 
         const pfc_batch_job J = jobs[R.job];
         batch_item<K>(J, it);
     }
+}
+
+// Clears the batch queue head (a kernel, not a memset node, in captured graphs).
+__global__ void k_zero_u32(uint32_t* p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+// Copies every slot's control block into one array, for one readback per
+// round: out[i] = *jobs[i].ctrl for i < *nslots.
+__global__ __launch_bounds__(256) void k_gather_ctrl(const pfc_batch_job* __restrict__ jobs,
+                                                     const uint32_t* __restrict__ nslots,
+                                                     pfc_ctrl* __restrict__ out) {
+    const uint32_t n = *nslots;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = *jobs[i].ctrl;
 }
 
 // ------------------------------------------------------------------ probe
@@ -1216,16 +1236,29 @@ int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint3
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_check launch");
 }
 
-int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, uint32_t nranges,
+int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, const uint32_t* d_nranges,
                  uint32_t* d_head, uint32_t ppl, uint32_t nblocks, hipStream_t st) {
-    if (nranges == 0) return PFC_OK;
     if (nblocks == 0) nblocks = 1;
-    if (ppl == 1) hipLaunchKernelGGL(k_batch<1>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_ranges, nranges, d_head);
-    else if (ppl == 2) hipLaunchKernelGGL(k_batch<2>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_ranges, nranges, d_head);
-    else if (ppl == 4) hipLaunchKernelGGL(k_batch<4>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_ranges, nranges, d_head);
+    if (ppl == 1) hipLaunchKernelGGL(k_batch<1>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_ranges, d_nranges, d_head);
+    else if (ppl == 2) hipLaunchKernelGGL(k_batch<2>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_ranges, d_nranges, d_head);
+    else if (ppl == 4) hipLaunchKernelGGL(k_batch<4>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_ranges, d_nranges, d_head);
     else return set_error(PFC_ERR_INVALID, "batch: primes_per_lane must be 1, 2 or 4");
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_batch launch");
+}
+
+int launch_zero_u32(uint32_t* d_p, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, d_p);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "k_zero_u32 launch");
+}
+
+int launch_gather_ctrl(const pfc_batch_job* d_jobs, const uint32_t* d_nslots, pfc_ctrl* d_out, uint32_t cap,
+                       hipStream_t st) {
+    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((cap + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_gather_ctrl, dim3(blocks), dim3(256), 0, st, d_jobs, d_nslots, d_out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "k_gather_ctrl launch");
 }
 
 int launch_verify_table(uint32_t kind, uint32_t N, const pfc_setup_table* t, uint64_t* d_work,

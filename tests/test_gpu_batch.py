@@ -166,3 +166,42 @@ def test_wide_composite_range_walks_bitmap(orc, dev):
     assert (plain.checksum, plain.primecount, plain.factors) == want
     [bat] = PK.run_batch([job], device=dev, verify_threads=8)
     assert (bat.checksum, bat.primecount, bat.factors) == want
+
+
+@pytest.mark.parametrize("ppl", [1, 2])
+def test_graph_captured_rounds_match_eager_and_oracle(orc, dev, ppl):
+    """Batcher rounds replayed from one captured graph (tuning.use_graph):
+    many short rounds (each a replay, with the rounds' own eager generation
+    launches and host synchronisations between replays), jobs added between
+    rounds (the slot count changes, so the graph is captured again), and a
+    second batch through the same device: every job's result equals the eager
+    batcher's and the oracle's."""
+    import pfcsieve_amd as PK
+    results = {}
+    for graph in (0, 1):
+        dev.set_tuning(use_graph=graph, primes_per_lane=ppl)
+        try:
+            b = PK.Batcher(dev)
+            jobs = [PK.Job(dev, *d) for d in MIX]
+            try:
+                rounds = 0
+                for k, j in enumerate(jobs):
+                    b.add(j)
+                    if k % 3 == 2:
+                        b.round(max_steps=150)
+                        rounds += 1
+                while b.pending():
+                    b.round(max_steps=150)
+                    rounds += 1
+                assert rounds > 10
+                results[graph] = [PK.job_result(j, verify_threads=8) for j in jobs]
+            finally:
+                b.destroy()
+                for j in jobs:
+                    j.destroy()
+        finally:
+            dev.set_tuning()
+    for d, e, g in zip(MIX, results[0], results[1]):
+        assert (g.checksum, g.primecount, g.factors) == (e.checksum, e.primecount, e.factors), d
+        cs, pc, fac = _ref(orc, d)
+        assert (g.checksum, g.primecount, g.factors) == (cs, pc, fac), d

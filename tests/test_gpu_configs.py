@@ -120,3 +120,22 @@ def test_served_stream_prefix_equals_standalone(dev):
         alone = PK.run_sieve(*r.desc, device=dev, verify_threads=8)
         got = r.result
         assert (got.checksum, got.primecount, got.factors) == (alone.checksum, alone.primecount, alone.factors), r.desc
+
+
+@pytest.mark.parametrize("desc", [
+    (5, 10**12, 10**12 + 300_000, 101, 10**6),                 # -! -c, the headline's n-range and p
+    (2, 2**62 - 4 * 10**9, 2**62 - 4 * 10**9 + 400_000, 101, 10**7),   # -#, the primorial job's n-range and p
+])
+def test_full_n_range_job_every_candidate(orc, dev, desc):
+    """The benchmark configurations' full n-ranges through the job path the
+    benchmark times (composite list, product steps below n = 65,536, paired
+    and, near 2^62, triple steps; launches of 2^18 steps) and through the batch
+    kernel, on p-windows of about 10^4 candidates: checksum, candidate count
+    and factor records of every candidate against the oracle."""
+    import pfcsieve_amd as PK
+    want = _oracle(orc, desc)
+    assert want[1] > 8000
+    got = PK.run_sieve(*desc, device=dev, verify_threads=8)
+    assert (got.checksum, got.primecount, got.factors) == want
+    [bat] = PK.run_batch([desc], device=dev, max_steps=300_000, verify_threads=8)
+    assert (bat.checksum, bat.primecount, bat.factors) == want
