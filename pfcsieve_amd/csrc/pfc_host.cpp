@@ -375,7 +375,7 @@ extern "C" int pfc_device_open(int ordinal, pfc_device** out) {
 
 extern "C" int pfc_device_close(pfc_device* dev) {
     if (!dev) return PFC_OK;
-    hipSetDevice(dev->ordinal);
+    (void)hipSetDevice(dev->ordinal);
     pool_flush(dev);
     for (hipStream_t s : dev->pool.streams) (void)hipStreamDestroy(s);
     for (void* slab : dev->pool.slabs) (void)hipHostFree(slab);
@@ -421,14 +421,20 @@ extern "C" int pfc_get_tuning(const pfc_device* dev, pfc_tuning* t) {
 
 extern "C" int pfc_device_malloc(pfc_device* dev, size_t bytes, void** ptr) {
     if (!dev || !ptr) return set_error(PFC_ERR_INVALID, "device_malloc: null pointer");
-    hipSetDevice(dev->ordinal);
+    {
+        const hipError_t se = hipSetDevice(dev->ordinal);
+        if (se != hipSuccess) return hip_fail(se, "device_malloc: hipSetDevice");
+    }
     hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
     return e == hipSuccess ? PFC_OK : hip_fail(e, "hipMalloc");
 }
 
 extern "C" int pfc_device_free(pfc_device* dev, void* ptr) {
     if (!dev) return set_error(PFC_ERR_INVALID, "device_free: null device");
-    hipSetDevice(dev->ordinal);
+    {
+        const hipError_t se = hipSetDevice(dev->ordinal);
+        if (se != hipSuccess) return hip_fail(se, "device_free: hipSetDevice");
+    }
     hipError_t e = hipFree(ptr);
     return e == hipSuccess ? PFC_OK : hip_fail(e, "hipFree");
 }
@@ -436,7 +442,10 @@ extern "C" int pfc_device_free(pfc_device* dev, void* ptr) {
 extern "C" int pfc_memcpy(pfc_device* dev, void* dst, const void* src, size_t bytes, int kind) {
     if (!dev || (!dst && bytes) || (!src && bytes)) return set_error(PFC_ERR_INVALID, "memcpy: null pointer");
     if (kind < 1 || kind > 3) return set_error(PFC_ERR_INVALID, "memcpy: kind must be 1, 2 or 3");
-    hipSetDevice(dev->ordinal);
+    {
+        const hipError_t se = hipSetDevice(dev->ordinal);
+        if (se != hipSuccess) return hip_fail(se, "memcpy: hipSetDevice");
+    }
     const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost
                                                                          : hipMemcpyDeviceToDevice;
     hipError_t e = hipMemcpy(dst, src, bytes, k);
@@ -445,20 +454,29 @@ extern "C" int pfc_memcpy(pfc_device* dev, void* dst, const void* src, size_t by
 
 extern "C" int pfc_device_synchronize(pfc_device* dev) {
     if (!dev) return set_error(PFC_ERR_INVALID, "synchronize: null device");
-    hipSetDevice(dev->ordinal);
+    {
+        const hipError_t se = hipSetDevice(dev->ordinal);
+        if (se != hipSuccess) return hip_fail(se, "synchronize: hipSetDevice");
+    }
     hipError_t e = hipDeviceSynchronize();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "hipDeviceSynchronize");
 }
 
 extern "C" int pfc_probe_copy(pfc_device* dev, uint64_t bytes, double* ms) {
     if (!dev) return set_error(PFC_ERR_INVALID, "probe_copy: null device");
-    hipSetDevice(dev->ordinal);
+    {
+        const hipError_t se = hipSetDevice(dev->ordinal);
+        if (se != hipSuccess) return hip_fail(se, "probe_copy: hipSetDevice");
+    }
     return pfc::probe_copy(bytes, ms);
 }
 
 extern "C" int pfc_probe_mul_rates(pfc_device* dev, double rates[3]) {
     if (!dev || !rates) return set_error(PFC_ERR_INVALID, "probe: null pointer");
-    hipSetDevice(dev->ordinal);
+    {
+        const hipError_t se = hipSetDevice(dev->ordinal);
+        if (se != hipSuccess) return hip_fail(se, "probe: hipSetDevice");
+    }
     return pfc::probe_mul_rates(dev->prop.multiProcessorCount, rates);
 }
 
@@ -591,10 +609,10 @@ void free_job_device(pfc_job* j) {
     pool_pinned_put(j->dev, j->h_bounds);
     if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
     if (j->graph) (void)hipGraphDestroy(j->graph);
-    if (j->ev0) hipEventDestroy(j->ev0);
-    if (j->ev1) hipEventDestroy(j->ev1);
-    if (j->evw0) hipEventDestroy(j->evw0);
-    if (j->evw1) hipEventDestroy(j->evw1);
+    if (j->ev0) (void)hipEventDestroy(j->ev0);
+    if (j->ev1) (void)hipEventDestroy(j->ev1);
+    if (j->evw0) (void)hipEventDestroy(j->evw0);
+    if (j->evw1) (void)hipEventDestroy(j->evw1);
     if (j->stream) pool_stream_put(j->dev, j->stream);
 }
 
@@ -730,7 +748,10 @@ extern "C" int pfc_job_create(pfc_device* dev, const pfc_job_desc* desc, pfc_job
     if (!dev || !out) return set_error(PFC_ERR_INVALID, "job_create: null pointer");
     int rc = validate(desc);
     if (rc) return rc;
-    hipSetDevice(dev->ordinal);
+    {
+        const hipError_t se = hipSetDevice(dev->ordinal);
+        if (se != hipSuccess) return hip_fail(se, "job_create: hipSetDevice");
+    }
     pfc_job* j = new pfc_job();
     j->dev = dev;
     j->desc = *desc;
@@ -1363,7 +1384,10 @@ int finish_segment(pfc_job* j, const pfc_ctrl& c, uint64_t seg_hi, uint64_t span
 extern "C" int pfc_batcher_create(pfc_device* dev, pfc_batcher** out) {
     pfc::g_err[0] = 0;
     if (!dev || !out) return set_error(PFC_ERR_INVALID, "batcher_create: null pointer");
-    hipSetDevice(dev->ordinal);
+    {
+        const hipError_t se = hipSetDevice(dev->ordinal);
+        if (se != hipSuccess) return hip_fail(se, "batcher_create: hipSetDevice");
+    }
     pfc_batcher* b = new pfc_batcher();
     b->dev = dev;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -1371,6 +1395,7 @@ extern "C" int pfc_batcher_create(pfc_device* dev, pfc_batcher** out) {
     if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_words, 4 * sizeof(uint32_t), hipHostMallocDefault);
     if (e != hipSuccess) {
         int rc = hip_fail(e, "batcher create");
+        if (b->d_words) (void)hipFree(b->d_words);
         if (b->stream) (void)hipStreamDestroy(b->stream);
         delete b;
         return rc;
@@ -1486,7 +1511,10 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
     pfc::g_err[0] = 0;
     if (!b || !ndone) return set_error(PFC_ERR_INVALID, "batcher_round: null pointer");
     *ndone = 0;
-    hipSetDevice(b->dev->ordinal);
+    {
+        const hipError_t se = hipSetDevice(b->dev->ordinal);
+        if (se != hipSuccess) return hip_fail(se, "batcher_round: hipSetDevice");
+    }
     hipStream_t st = b->stream;
     if (max_steps == 0) max_steps = 0xFFFFFFFFu;
     const size_t ns = b->slots.size();
@@ -1659,7 +1687,7 @@ extern "C" int pfc_batcher_round_work(pfc_batcher* b, uint64_t max_updates, pfc_
 
 extern "C" int pfc_batcher_destroy(pfc_batcher* b) {
     if (!b) return PFC_OK;
-    hipSetDevice(b->dev->ordinal);
+    (void)hipSetDevice(b->dev->ordinal);
     (void)hipStreamSynchronize(b->stream);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
     if (b->d_ranges) (void)hipFree(b->d_ranges);
