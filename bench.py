@@ -53,8 +53,8 @@ PRODUCT_LIMIT = 65536              # pairs with both multipliers up to this are 
 # the copy-kernel calibration, and WRITE_SIZE, of k_walk: p, v and both
 # residues read, both residues written, plus the launch's composite-list
 # reads amortised over its candidates).
-WALK_FETCH_PER_CAND = 28.90
-WALK_WRITE_PER_CAND = 16.00
+WALK_FETCH_PER_CAND = 28.19
+WALK_WRITE_PER_CAND = 16.50
 
 HEAD_MODES, HEAD_PMIN, HEAD_PMAX, HEAD_NMIN, HEAD_NMAX = 5, 10**12, 10**12 + 2 * 10**9, 101, 10**6
 
@@ -335,6 +335,26 @@ def kernel_rates(dev, reps=3):
     return out
 
 
+def walk_modes(dev):
+    """One 2^24-candidate p-segment of each walk mode at the shapes the mixes
+    use: -! and -c alone at n in [101, 1e6) from p = 1e12, and -# at
+    n in [101, 1e7) from p = 2^62 - 4e9 (walk launches timed by HIP events)."""
+    import pfcsieve_amd as P
+    from pfcsieve_amd import workload as Wl
+    out = {}
+    for name, modes, pmin, nmin, nmax in (("factorial", 1, 10**12, 101, 10**6), ("compositorial", 4, 10**12, 101, 10**6),
+                                          ("primorial", 2, 2**62 - 4 * 10**9, 101, 10**7)):
+        j = P.Job(dev, modes, pmin, pmin + 4 * 10**9, nmin, nmax)
+        j.run(1)
+        s = j.stats()
+        j.destroy()
+        out[name] = {"workload": f"modes {modes}, n in [{nmin}, {nmax}), one p-segment from {pmin}",
+                     "candidates": s.primecount, "steps_per_candidate": Wl.walk_steps(Wl.Unit(modes, 0, 0, nmin, nmax)),
+                     "walk_launches": s.walk_launches, "walk_ms": s.walk_ms,
+                     "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
+    return out
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -393,6 +413,9 @@ def main():
     ap.add_argument("--serve-round-updates", type=float, default=4e11)
     ap.add_argument("--no-headline", action="store_true")
     ap.add_argument("--kernel-rates", action="store_true", help="also time k_generate and k_setup standalone")
+    ap.add_argument("--modes", action="store_true", help="also time one p-segment of each walk mode")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="run one plain device copy of this many bytes first (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-width", type=int, default=1_000_000)
     ap.add_argument("--progress", action="store_true", help="progress lines on stderr")
@@ -405,6 +428,8 @@ def main():
 
     import pfcsieve_amd as P
     dev = P.Device(0)
+    if args.copy_probe:
+        dev.probe_copy(args.copy_probe)
     rates = dev.probe_mul_rates()
     head = None if args.no_headline else measure_headline(dev, K, W, args.progress)
     mix = measure_mix(dev, K, W, args.progress, args.mix_units) if args.mix_units else None
@@ -443,6 +468,8 @@ def main():
     line["config"]["small_job"] = small_job(dev)
     if args.kernel_rates and "kernel_rates" not in line:
         line["kernel_rates"] = kernel_rates(dev)
+    if args.modes:
+        line["walk_modes"] = walk_modes(dev)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_width)
         if head:
