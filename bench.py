@@ -31,7 +31,9 @@ Workloads (synthetic: every candidate prime is generated on the device):
   stream) against 256 CUs x 64 lanes x 2.4 GHz.
 * ``served`` -- a prefix of the seeded served stream (splitmix64 seed 2024,
   units drawn as in the mix), Poisson arrivals at 0.8x the capacity that the
-  batched mix's measured work rate gives for those jobs.
+  prefix's own device-work bound gives at the kernels' measured rates (walk,
+  set-up, generation), admitted by HBM footprint, rounds of about 4e11 p*n
+  updates; throughput and p50/p99 job latency.
 * ``cpu_baseline`` -- the CPU oracle (OpenMP over candidates) on a bounded
   p-sample of the headline job, plus the reference's CPU-runnable case in
   full.
@@ -257,23 +259,29 @@ def setup_products(u):
     return _TABLE_COST[key]
 
 
-def serve_prefix(dev, njobs, mix, round_updates, progress):
+def stream_bound(units, rates):
+    """Device-work bound of a list of units at the kernels' measured rates:
+    walk updates at rates["walk"], start-residue products at rates["setup"],
+    candidates at rates["generate"] (candidate counts from the prime density
+    1/ln p)."""
+    t = 0.0
+    for u in units:
+        c = (u.pmax - u.pmin) / math.log(u.pmin)
+        t += c * setup_products(u) / rates["setup"] + c / rates["generate"]
+        t += _unit_updates(u) / rates["walk"]
+    return t
+
+
+def serve_prefix(dev, njobs, rates, round_updates, progress):
     """The first njobs of the seeded 10,000-job stream through the serving
-    loop.  Arrival rate: 0.8 x the capacity the batched mix's measured work
-    rate gives for these jobs (work = walk updates + set-up products x the
-    mix's product/update time ratio), instead of a separate capacity run."""
+    loop.  Poisson arrivals at 0.8 x the capacity the stream's own device-work
+    bound gives (stream_bound at the measured kernel rates), instead of a
+    separate capacity run."""
     import pfcsieve_amd.serve as S
     from pfcsieve_amd import workload as Wl
     units = Wl.mix(10_000, seed=2024)[:njobs]
-    # the mix's device time split: walk at the headline walk rate, the rest
-    # set-up; one work unit = one walk update
-    upd_s = mix["batched_s"]
-    ratio = 2.25                                       # set-up product / walk update time, measured on MI355X
-    mix_work = mix["walk_updates"] + ratio * mix["setup_products"]
-    rate = mix_work / upd_s
-    work = sum(_unit_updates(u) + ratio * setup_products(u) * (u.pmax - u.pmin) / math.log(u.pmin)
-               for u in units)
-    lam = 0.8 * njobs / (work / rate)
+    bound = stream_bound(units, rates)
+    lam = 0.8 * njobs / bound
     arrivals = Wl.poisson_arrivals(njobs, lam, seed=777)
     info = dev.info()
     say = (lambda m: _progress(True, m)) if progress else None
@@ -285,9 +293,12 @@ def serve_prefix(dev, njobs, mix, round_updates, progress):
     upd = sum(r.result.primecount * Wl.walk_steps(units[r.rid]) for r in srv.done)
     return {"jobs": st.jobs, "stream": f"first {njobs} of 10,000 seeded jobs (splitmix64 seed 2024, units as in "
                                        f"the mix), Poisson arrivals (seed 777)",
-            "arrival_rate_per_s": lam, "wall_s": st.wall_s, "rounds": st.rounds, "updates_per_s": upd / st.wall_s,
-            "p50_latency_s": st.pct(50), "p99_latency_s": st.pct(99), "peak_resident_bytes": st.peak_resident_bytes,
-            "round_policy": f"about {round_updates:.3g} p*n updates per round"}
+            "device_work_bound_s": bound, "arrival_rate_per_s": lam, "last_arrival_s": arrivals[-1],
+            "wall_s": st.wall_s, "rounds": st.rounds, "updates_per_s": upd / st.wall_s,
+            "jobs_per_s": st.jobs / st.wall_s, "p50_latency_s": st.pct(50), "p99_latency_s": st.pct(99),
+            "max_latency_s": max(st.latencies) if st.latencies else None,
+            "peak_resident_bytes": st.peak_resident_bytes, "round_policy": f"about {round_updates:.3g} p*n "
+                                                                           f"updates per round"}
 
 
 def kernel_rates(dev, reps=3):
@@ -463,12 +474,23 @@ def main():
         mix["device_work_bound_s"] = bound
         mix["batched_frac_of_bound"] = bound / mix["batched_s"]
         mix["plain_frac_of_bound"] = bound / mix["plain_s"]
-    if mix and args.serve_jobs:
-        line["served"] = serve_prefix(dev, args.serve_jobs, mix, int(args.serve_round_updates), args.progress)
+    if args.serve_jobs:
+        if "kernel_rates" not in line:
+            line["kernel_rates"] = kernel_rates(dev)
+        if head:
+            walk = head["walk_updates_per_s"]
+        else:
+            line["walk_modes"] = line.get("walk_modes") or walk_modes(dev)
+            walk = sum(v["walk_updates_per_s"] for v in line["walk_modes"].values()) / len(line["walk_modes"])
+        kr = line["kernel_rates"]
+        rates = {"walk": walk, "setup": kr["setup_nmin_1e6"]["products_per_s"],
+                 "generate": kr["generate_p1e12_2e9"]["candidates_per_s"]}
+        line["served"] = serve_prefix(dev, args.serve_jobs, rates, int(args.serve_round_updates), args.progress)
+        line["served"]["rates"] = rates
     line["config"]["small_job"] = small_job(dev)
     if args.kernel_rates and "kernel_rates" not in line:
         line["kernel_rates"] = kernel_rates(dev)
-    if args.modes:
+    if args.modes and "walk_modes" not in line:
         line["walk_modes"] = walk_modes(dev)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_width)
