@@ -25,10 +25,11 @@ Workloads (synthetic: every candidate prime is generated on the device):
   launch; generate and set-up run before the first, the checksum fold after
   the last); the whole job is timed.  Warmup: the same n-range on
   p in [1e12 - 1e8, 1e12), walked in W launches.
-* ``roofline`` -- the walk kernel (k_walk) of the headline job: multiply
-  instructions it issues (v_mad_u64_u32, counted from its instruction
-  sequence per launch) per second of launch time (HIP events on the job's
-  stream) against 256 CUs x 64 lanes x 2.4 GHz.
+* ``roofline`` -- the walk kernel (k_walk) of the headline job: INT32
+  multiply instructions it issues (counted from its instruction sequence per
+  launch: five per update in the narrow form the headline takes) per second
+  of launch time (HIP events on the job's stream) against 256 CUs x 64 lanes
+  x 2.4 GHz.
 * ``served`` -- a prefix of the seeded served stream (splitmix64 seed 2024,
   units drawn as in the mix), Poisson arrivals at 0.8x the capacity that the
   prefix's own device-work bound gives at the kernels' measured rates (walk,
@@ -76,32 +77,54 @@ def prime_flags(n):
     return s
 
 
-def walk_mads(nmin, nmax, chunk):
-    """v_mad_u64_u32 instructions per candidate that the F|C walk of
+def narrow_fits(p, bmax):
+    """The walk's narrow form (plain residues below 2p, five multiplies per
+    step, pfc::narrow_fits): 2^34 < p < 2^55, 1 <= bmax < 2^22, p * bmax < 2^61."""
+    return 2**34 < p < 2**55 and 1 <= bmax < 2**22 and p * bmax < 2**61
+
+
+def walk_muls(nmin, nmax, chunk, pmax):
+    """INT32 multiply instructions per candidate that the F|C walk of
     n in [nmin, nmax) issues in launches of `chunk` steps, with every step
-    paired (all p above twice the largest n): five per multiplier step, five
-    per pair taken as one step by its product (factorial trips of four whose
-    last n is at most PRODUCT_LIMIT; composite-list blocks of 16, aligned to
-    the list, whose last composite is).  Returns (mads, updates, product
-    updates) per candidate."""
+    paired (all p above twice the largest n), for candidates up to pmax.
+    A launch whose largest multiplier fits the narrow form with pmax takes
+    five per update (v_mul_lo_u32, v_mad_u64_u32, v_mul_hi_u32,
+    v_mad_i32_i24, v_mad_u64_u32).  Otherwise, for pmax below 2^63, the
+    medium form takes six per multiplier step (v_mul_lo_u32, v_mad_u64_u32,
+    two v_mul_hi_u32, v_mul_lo_u32, v_mad_u64_u32), a pair with product
+    below 2^28 being one step; above it the shifted 3-by-2 step issues five
+    v_mad_u64_u32 per multiplier step and per product step of two
+    multipliers up to PRODUCT_LIMIT.  Pairs are the factorial trips of four
+    and the composite-list blocks of 16, aligned to the list.  Returns
+    (multiplies, updates, product updates, narrow updates) per candidate."""
     import numpy as np
     isp = prime_flags(nmax)
     comp = np.nonzero(~isp[nmin:nmax])[0] + nmin          # composites in [nmin, nmax), the C list
-    prod = 0
+    medium = pmax < 2**63
+    per_step = 6.0 if medium else 5.0
+    muls = prod = narrow = 0.0
     for a in range(nmin, nmax, chunk):
         b = min(nmax, a + chunk)
-        jp = min(b, PRODUCT_LIMIT + 1)
-        if jp > a:
-            prod += (jp - a) // 4 * 4
         c0 = int(np.searchsorted(comp, a))
         c1 = int(np.searchsorted(comp, b))
+        if narrow_fits(pmax, b - 1):
+            narrow += (b - a) + (c1 - c0)
+            muls += 5.0 * ((b - a) + (c1 - c0))
+            continue
+        lim = 16383 if medium else PRODUCT_LIMIT           # medium: m * n < 2^28
+        pp = 0
+        jp = min(b, lim + 1)
+        if jp > a:
+            pp += (jp - a) // 4 * 4
         j = min(c1, (c0 + 15) & ~15)
         while j + 16 <= c1:
-            if comp[j + 15] <= PRODUCT_LIMIT:
-                prod += 16
+            if comp[j + 15] <= lim:
+                pp += 16
             j += 16
+        prod += pp
+        muls += per_step * ((b - a) + (c1 - c0)) - per_step / 2 * pp
     upd = (nmax - nmin) + len(comp)
-    return 5.0 * upd - 2.5 * prod, upd, prod
+    return muls, upd, prod, narrow
 
 
 def measure_headline(dev, K, W, progress):
@@ -144,11 +167,11 @@ def measure_headline(dev, K, W, progress):
 
 def roofline(head, mul_rates):
     """Roofline figures of k_walk on the headline job."""
-    mads, upd, prod = walk_mads(HEAD_NMIN, HEAD_NMAX, head["walk_chunk"])
+    muls, upd, prod, narrow = walk_muls(HEAD_NMIN, HEAD_NMAX, head["walk_chunk"], HEAD_PMAX - 1)
     launches = head["walk_launches"]
     avg_s = head["walk_ms"] * 1e-3 / launches
     cand = head["candidates_timed"]
-    achieved = mads * cand / launches / avg_s
+    achieved = muls * cand / launches / avg_s
     ups = head["walk_updates_per_s"]
     mad_rate = mul_rates["mad_u64_u32"]
     return {
@@ -156,19 +179,20 @@ def roofline(head, mul_rates):
         "achieved": achieved / 1e12, "peak": PEAK_LANE_OPS / 1e12, "unit": "Tops/s",
         "frac": achieved / PEAK_LANE_OPS,
         "traffic": (WALK_FETCH_PER_CAND + WALK_WRITE_PER_CAND) * cand,
-        "mads_per_update": mads / upd, "updates_per_launch": upd * cand / launches,
+        "mul_per_update": muls / upd, "narrow_share": narrow / upd, "updates_per_launch": upd * cand / launches,
         "avg_launch_ms": avg_s * 1e3,
         "frac_of_measured_mad_rate": achieved / mad_rate,
-        "issue": "the walk is bound by VALU issue, not by its multiplies: 5 v_mad_u64_u32 per update share "
-                 "the issue slots with the 64-bit add, the quotient compare and correction add, the band "
-                 "test and two register moves (rocprof VALU and SALU counts per update in profiles/)",
+        "issue": "the walk is bound by VALU issue: in the narrow form (p * n < 2^61) each update is five "
+                 "32-bit multiply instructions, and every pair of updates adds a compare and a masked add "
+                 "(the reduction below p) and a 64-bit add and compare (the band test); rocprof VALU and "
+                 "SALU counts per update in profiles/",
         "reference_equivalent": {"mul_per_update": W_MUL_REF, "achieved": ups * W_MUL_REF / 1e12,
                                  "frac": ups * W_MUL_REF / PEAK_LANE_OPS,
                                  "note": "updates/s priced at the reference's 19 INT32 multiplies per update "
                                          "(its 64-bit Montgomery product); not instructions this walk issues"},
         "measured_lane_rates": {k: v / 1e12 for k, v in mul_rates.items()},
-        "note": "achieved = v_mad_u64_u32 issued per launch (5 per multiplier step, 5 per product step of "
-                "two multipliers) / average k_walk launch duration (HIP events on the job's stream); peak = "
+        "note": "achieved = INT32 multiply instructions issued per launch (walk_muls: 5 per update in the "
+                "narrow form) / average k_walk launch duration (HIP events on the job's stream); peak = "
                 "256 CU x 64 lanes x 2.4 GHz; traffic = HBM bytes per launch (rocprof FETCH_SIZE x copy "
                 "calibration + WRITE_SIZE, per candidate, x candidates per launch)",
     }
@@ -240,8 +264,10 @@ def measure_mix(dev, K, W, progress, nunits=256):
         dev.set_tuning()
     cand = [r[1] for r in batched]
     upd = sum(c * s for c, s in zip(cand, steps))
+    upd_narrow = sum(c * narrow_steps(u) for c, u in zip(cand, units))
     products = sum(c * setup_products(u) for c, u in zip(cand, units))
-    return {"units": len(units), "candidates": sum(cand), "walk_updates": upd, "setup_products": products,
+    return {"units": len(units), "candidates": sum(cand), "walk_updates": upd, "walk_updates_narrow": upd_narrow,
+            "setup_products": products,
             "segments": sum(s.segments for s in stats), "plain_s": t_plain, "batched_s": t_batch,
             "rounds": rounds, "plain_updates_per_s": upd / t_plain, "batched_updates_per_s": upd / t_batch,
             "batched_over_plain": t_plain / t_batch, "results_identical": plain == batched}
@@ -259,16 +285,32 @@ def setup_products(u):
     return _TABLE_COST[key]
 
 
+def narrow_steps(u):
+    """Walk steps per candidate of unit u that the narrow form takes for
+    every p of the unit: those whose multiplier n satisfies
+    narrow_fits(pmax - 1, n) (launch boundaries ignored)."""
+    from pfcsieve_amd import workload as Wl
+    pmax = u.pmax - 1
+    if not (2**34 < u.pmin and pmax < 2**55):
+        return 0
+    nlim = min(2**22 - 1, (2**61 - 1) // pmax)
+    hi = min(u.nmax, nlim + 1)
+    return Wl.walk_steps(Wl.Unit(u.modes, u.pmin, u.pmax, u.nmin, hi)) if hi > u.nmin else 0
+
+
 def stream_bound(units, rates):
     """Device-work bound of a list of units at the kernels' measured rates:
-    walk updates at rates["walk"], start-residue products at rates["setup"],
+    walk updates at rates["walk_narrow"] where the narrow form applies and
+    rates["walk"] (the medium form) elsewhere, start-residue products at rates["setup"],
     candidates at rates["generate"] (candidate counts from the prime density
     1/ln p)."""
+    from pfcsieve_amd import workload as Wl
     t = 0.0
     for u in units:
         c = (u.pmax - u.pmin) / math.log(u.pmin)
         t += c * setup_products(u) / rates["setup"] + c / rates["generate"]
-        t += _unit_updates(u) / rates["walk"]
+        steps, nsteps = Wl.walk_steps(u), narrow_steps(u)
+        t += c * (steps - nsteps) / rates["walk"] + c * nsteps / rates["walk_narrow"]
     return t
 
 
@@ -332,6 +374,16 @@ def kernel_rates(dev, reps=3):
                      "integers_per_s": span / best}
         d_p.free()
         ctrl.free()
+    # The medium walk (p * n above 2^61, p below 2^63): -! -c, n in [9e5, 1e6),
+    # p from 2^61.
+    j = P.Job(dev, 5, 2**61, 2**61 + 5 * 10**8, 900_000, 10**6)
+    j.run()
+    s = j.stats()
+    j.destroy()
+    out["walk_medium_p2e61"] = {"workload": "-! -c n [9e5, 1e6) p in [2^61, 2^61 + 5e8): the medium walk "
+                                            "(p * n > 2^61, p < 2^63), walk launches timed by HIP events",
+                                 "candidates": s.primecount, "walk_updates": s.walk_updates,
+                                 "walk_ms": s.walk_ms, "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
     products = P.setup_table_cost(0, 10**6 - 1)
     j = P.Job(dev, 1, 10**15, 10**15 + 2 * 10**9, 10**6, 10**6 + 256)
     j.run(1)
@@ -383,7 +435,7 @@ def cpu_baseline(width):
     (-! n [101, 5000) p [1e9, 1.001e9)) in full."""
     import oracle as O
     O.build()
-    steps = walk_mads(HEAD_NMIN, HEAD_NMAX, HEAD_NMAX)[1]
+    steps = walk_muls(HEAD_NMIN, HEAD_NMAX, HEAD_NMAX, HEAD_PMAX - 1)[1]
     t = time.perf_counter()
     r = O.run(HEAD_MODES, HEAD_PMIN, HEAD_PMIN + width, HEAD_NMIN, HEAD_NMAX)
     dt = time.perf_counter() - t
@@ -465,10 +517,15 @@ def main():
         line["kernel_rates"] = kernel_rates(dev)
     if head and mix:
         # The mix's device work at the kernels' own measured rates: walk updates
-        # at the headline walk rate, start-residue products at the set-up rate,
-        # candidates at the generator's rate on the 1e12 window.
+        # at the headline walk rate (the narrow form) where the narrow form
+        # applies and at the medium walk's rate elsewhere (every p of the mix
+        # is below 2^63), start-residue
+        # products at the set-up rate, candidates at the generator's rate on
+        # the 1e12 window.
         kr = line["kernel_rates"]
-        bound = (mix["walk_updates"] / head["walk_updates_per_s"]
+        medium = kr["walk_medium_p2e61"]["walk_updates_per_s"]
+        bound = (mix["walk_updates_narrow"] / head["walk_updates_per_s"]
+                 + (mix["walk_updates"] - mix["walk_updates_narrow"]) / medium
                  + mix["setup_products"] / kr["setup_nmin_1e6"]["products_per_s"]
                  + mix["candidates"] / kr["generate_p1e12_2e9"]["candidates_per_s"])
         mix["device_work_bound_s"] = bound
@@ -477,13 +534,14 @@ def main():
     if args.serve_jobs:
         if "kernel_rates" not in line:
             line["kernel_rates"] = kernel_rates(dev)
+        kr = line["kernel_rates"]
         if head:
-            walk = head["walk_updates_per_s"]
+            walk_narrow = head["walk_updates_per_s"]
         else:
             line["walk_modes"] = line.get("walk_modes") or walk_modes(dev)
-            walk = sum(v["walk_updates_per_s"] for v in line["walk_modes"].values()) / len(line["walk_modes"])
-        kr = line["kernel_rates"]
-        rates = {"walk": walk, "setup": kr["setup_nmin_1e6"]["products_per_s"],
+            walk_narrow = line["walk_modes"]["factorial"]["walk_updates_per_s"]
+        rates = {"walk": kr["walk_medium_p2e61"]["walk_updates_per_s"], "walk_narrow": walk_narrow,
+                 "setup": kr["setup_nmin_1e6"]["products_per_s"],
                  "generate": kr["generate_p1e12_2e9"]["candidates_per_s"]}
         line["served"] = serve_prefix(dev, args.serve_jobs, rates, int(args.serve_round_updates), args.progress)
         line["served"]["rates"] = rates

@@ -82,6 +82,10 @@ typedef struct pfc_tuning {
                                   (pfc_stage_verify_table); 0 (default): check them */
     uint32_t lds_segment;      /* integers sieved per generator workgroup and staged in LDS:
                                   15360 (default), 30720 or 61440 */
+    uint32_t walk_forms_off;   /* walk arithmetic forms not to take (for comparisons): bit 0 the
+                                  narrow form (plain residues, 2^34 < p < 2^55, p * n < 2^61), bit 1
+                                  the medium form (2^32 < p < 2^63, n < 2^28); 0 (default): each
+                                  launch takes the cheapest form its primes and multipliers allow */
 } pfc_tuning;
 
 /* Host-side summary of a finished job (cl_sieve.cpp:1748). */

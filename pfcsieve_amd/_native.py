@@ -40,7 +40,8 @@ VERIFY_WORDS = 2 + 256  # PFC_VERIFY_WORDS
 class Tuning(C.Structure):
     _fields_ = [("primes_per_lane", C.c_uint32), ("block_size", C.c_uint32),
                 ("walk_chunk", C.c_uint32), ("seg_candidates", C.c_uint32), ("use_graph", C.c_uint32),
-                ("skip_table_check", C.c_uint32), ("lds_segment", C.c_uint32)]
+                ("skip_table_check", C.c_uint32), ("lds_segment", C.c_uint32),
+                ("walk_forms_off", C.c_uint32)]
 
 
 class JobStats(C.Structure):

@@ -137,6 +137,139 @@ PFC_HD bool walk_attention(uint64_t x, uint32_t c, uint32_t thr) { return (uint3
 PFC_HD uint32_t walk_k1(uint32_t d1, uint32_t d0) { return d0 ? ~d1 : 0u - d1; }
 PFC_HD uint32_t walk_k0(uint32_t d0) { return 0u - d0; }
 
+// ---------------------------------------------------------------- narrow walk
+
+// Walk arithmetic for moduli with 2^34 < p < 2^55 and multipliers n with
+// p * n < 2^61 and n < 2^22 ("narrow" moduli; the headline's p ~ 10^12 with n
+// below 10^6 is one).  Residues are plain integers X in [0, 2p), not reduced
+// and not shifted.  Then t = X * n < 2^62 is one 64-bit word and a quotient
+// estimate from its high word alone leaves a remainder below 2p, so a step is
+// five multiplies and no correction:
+//   t  = X0 * n + ((X1 * n) << 32)     X1 * n < 2^30 (one multiply, one
+//                                      multiply-add on the pair <X1*n, 0>)
+//   q  = hi32(t1 * m)                  m = floor(2^64 / p) (or one less)
+//   r  = t - q * p = q * np0 + <t1 - q * (p1 + 1), t0>   (mod 2^64)
+// with t1, t0 the words of t, p1 the high word of p and np0 = 2^32 - p0
+// (p0 != 0: p is odd).  The correction term is exact: with
+// 2^64 / p = m + phi, 0 <= phi < 2 (m may be one below the floor),
+//   t / p - q < 1 + t1 * phi / 2^32 + t0 / p < 1 + 2^-1 + 2^-2,
+// because t1 < 2^30 and t0 < 2^32 < p / 4; so floor(t / p) <= q + 1 and
+// 0 <= r < 2p.  q < 2n < 2^23 and p1 + 1 <= 2^23, so q * (p1 + 1) fits the
+// signed 24-bit multiply-add (v_mad_i32_i24) on the device.
+PFC_HD bool narrow_fits(uint64_t p, uint32_t bmax) {
+    if (bmax == 0 || bmax >= (1u << 22) || p <= (1ull << 34) || p >= (1ull << 55)) return false;
+    const uint64_t a = (p >> 32) * bmax;                       // < 2^45
+    if (a >= (1ull << 29)) return false;
+    return (a << 32) + (uint64_t)(uint32_t)p * bmax < (1ull << 61);
+}
+
+// floor(2^64 / p) from the 3-by-2 reciprocal of d = p << s (s < 30 here):
+// v + 2^32 = floor((2^96 - 1) / d) = floor(2^96 / d) (d does not divide 2^96),
+// and floor(floor(x) / 2^(32-s)) = floor(x / 2^(32-s)) = floor(2^64 / p).
+PFC_HD uint32_t narrow_m(uint32_t v, uint32_t s) { return (uint32_t)(((1ull << 32) + v) >> (32 - s)); }
+
+// The per-modulus constants of narrow_mul.
+PFC_HD uint32_t narrow_np0(uint64_t p) { return 0u - (uint32_t)p; }
+PFC_HD uint32_t narrow_kp1(uint64_t p) { return 0u - ((uint32_t)(p >> 32) + 1u); }
+
+PFC_HD uint32_t mulhi32(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(a, b);
+#else
+    return (uint32_t)(((uint64_t)a * b) >> 32);
+#endif
+}
+
+// X * n mod p up to one p, for X < 2p: a value in [0, 2p) congruent to X * n.
+PFC_HD uint64_t narrow_mul(uint64_t x, uint32_t n, uint32_t m, uint32_t np0, uint32_t kp1) {
+    uint32_t a = (uint32_t)(x >> 32) * n;
+#if defined(__HIP_DEVICE_COMPILE__)
+    // The empty asm statements keep the optimiser from recognising the
+    // 64-by-32 product and rebuilding it as a 96-bit one (with register
+    // moves around it): t is the multiply-add on the pair <a, 0>, and w lands
+    // in t's high register, so that <w, t0> is again a pair.
+    asm volatile("" : "+v"(a));
+#endif
+    uint64_t t = (uint64_t)(uint32_t)x * n + ((uint64_t)a << 32);
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(t));
+#endif
+    const uint32_t t1 = (uint32_t)(t >> 32);
+    const uint32_t q = mulhi32(t1, m);
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t w = (uint32_t)(__mul24((int)q, (int)kp1) + (int)t1);
+    asm volatile("" : "+v"(w));
+#else
+    const uint32_t w = t1 + q * kp1;
+#endif
+    return (uint64_t)q * np0 + (((uint64_t)w << 32) | (uint32_t)t);
+}
+
+// The narrow walk's test of a reduced residue x in [0, p) against a band B:
+// pass iff x <= B or x >= p - B, as x + c >= thr with c = 2^64 - B - 1 and
+// thr = p - 2B - 1 (one 64-bit add and one 64-bit compare; needs p > 2B + 1).
+PFC_HD void narrow_band(uint64_t p, uint64_t B, uint64_t* c, uint64_t* thr) {
+    *c = ~B;
+    *thr = p - 2 * B - 1;
+}
+PFC_HD bool narrow_attention(uint64_t x, uint64_t c, uint64_t thr) { return x + c >= thr; }
+
+// ---------------------------------------------------------------- medium walk
+
+// Walk arithmetic for 2^32 < p < 2^63 and multipliers n < 2^28 ("medium"
+// moduli, the ones outside the narrow form but for the top bit).  Residues
+// are kept at half the shifted scale: D = p << (s - 1) = d / 2 lies in
+// (2^62, 2^63), and x in [0, 2D) stands for X = x >> (s - 1).  With
+// M = floor(2^94 / D) (in (2^31, 2^32)), the quotient estimate
+//   f = hi32(x1 * M),   q = hi32(f * 4n)        (x1 = x >> 32)
+// is floor(x * n / D) or one less: f <= x * 2^30 / D < f + 3 (x's low word,
+// M's truncation and the floor each lose less than one), so
+//   x * n / D - q < 1 + 3n / 2^30 < 2   for n < 2^28.
+// The remainder r = x * n - q * D < 2D needs only the low 64 bits of x * n:
+//   T = x0 * n + ((x1 * n) << 32),  r = q * nD0 + <T1 - q * (D1 + 1), T0>
+// (mod 2^64), with nD0 = 2^32 - D0 (D0 != 0: its lowest set bit is bit
+// s - 1 < 31).  Seven instructions per step: v_mul_lo_u32 and v_mad_u64_u32
+// (T), two v_mul_hi_u32 (f, q), v_mul_lo_u32 and v_add_u32 (T1 - q(D1+1)),
+// v_mad_u64_u32 (r).
+PFC_HD bool medium_fits(uint64_t p, uint32_t bmax) {
+    return bmax != 0 && bmax < (1u << 28) && p > (1ull << 32) && p < (1ull << 63);
+}
+
+// M = floor(2^94 / D) from the 3-by-2 reciprocal of d = 2D:
+// v + 2^32 = floor(2^96 / d) = floor(2^95 / D), and halving it floors again.
+PFC_HD uint32_t medium_m(uint32_t v) { return (uint32_t)(((1ull << 32) + v) >> 1); }
+
+// x * n mod D up to one D, for x < 2D and n < 2^28 (n4 = 4n): a value in
+// [0, 2D) congruent to x * n; k1 = -(D1 + 1) mod 2^32, nd0 = 2^32 - D0.
+PFC_HD uint64_t medium_mul(uint64_t x, uint32_t n, uint32_t n4, uint32_t M, uint32_t nd0, uint32_t k1) {
+    const uint32_t x1 = (uint32_t)(x >> 32);
+    uint32_t a = x1 * n;
+#if defined(__HIP_DEVICE_COMPILE__)
+    // As in narrow_mul: T is the multiply-add on the pair <a, 0>, and the
+    // corrected high word is written back into T's high register.
+    asm volatile("" : "+v"(a));
+#endif
+    uint64_t T = (uint64_t)(uint32_t)x * n + ((uint64_t)a << 32);    // x * n mod 2^64
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(T));
+#endif
+    const uint32_t f = mulhi32(x1, M);
+    const uint32_t q = mulhi32(f, n4);
+#if defined(__HIP_DEVICE_COMPILE__)
+    // T + ((q * k1) << 32) as one 64-bit add on the pair <q * k1, 0> (built
+    // like <a, 0> above); left to itself the optimiser fuses q * k1 into a
+    // 64-bit multiply-add and reassembles the words with register moves.
+    uint32_t qk = q * k1;
+    asm volatile("" : "+v"(qk));
+    uint64_t Tw;
+    asm("v_lshl_add_u64 %0, %1, 0, %2" : "=v"(Tw) : "v"(T), "v"((uint64_t)qk << 32));
+    return (uint64_t)q * nd0 + Tw;
+#else
+    const uint32_t w = (uint32_t)(T >> 32) + q * k1;
+    return (uint64_t)q * nd0 + (((uint64_t)w << 32) | (uint32_t)T);
+#endif
+}
+
 // (a * n) mod d for a < d and any 32-bit n.  Keeps the shifted form.
 PFC_HD uint64_t mulmod32(uint64_t a, uint32_t n, const Modulus& m) {
     bool fix;

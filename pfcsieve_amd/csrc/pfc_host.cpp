@@ -350,6 +350,7 @@ static pfc_tuning default_tuning() {
     t.use_graph = 0;
     t.skip_table_check = 0;
     t.lds_segment = 15360;   // 256 threads x one 60-integer window
+    t.walk_forms_off = 0;
     return t;
 }
 
@@ -408,6 +409,7 @@ extern "C" int pfc_set_tuning(pfc_device* dev, const pfc_tuning* t) {
     if (!n.lds_segment) n.lds_segment = d.lds_segment;
     if (n.lds_segment != 15360 && n.lds_segment != 30720 && n.lds_segment != 61440)
         return set_error(PFC_ERR_INVALID, "lds_segment must be 15360, 30720 or 61440");
+    if (n.walk_forms_off > 3) return set_error(PFC_ERR_INVALID, "walk_forms_off must be 0 to 3");
     std::lock_guard<std::mutex> g(dev->mu);
     dev->tuning = n;
     return PFC_OK;
@@ -878,7 +880,8 @@ static int build_graph(pfc_job* j) {
         const uint32_t b = launch_end(j, (uint32_t)a, (uint32_t)std::min<uint64_t>(j->jend, a + chunk));
         rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, j->d_clist, j->d_cprefix, d.nmin,
                               j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
-                              cap, j->tuning.primes_per_lane, j->tuning.block_size, walk_bound(j, b), st);
+                              cap, j->tuning.primes_per_lane, j->tuning.block_size, walk_bound(j, b),
+                              j->tuning.walk_forms_off, st);
     }
     if (!rc)
         rc = pfc::launch_check(d.modes, d.nmax, j->d_p, j->d_v, j->d_one, j->d_ra, j->d_rb, j->d_ctrl,
@@ -933,7 +936,8 @@ static int run_segment(pfc_job* j, uint64_t lo, uint64_t hi, bool* overflow) {
         const uint32_t b = launch_end(j, (uint32_t)a, (uint32_t)std::min<uint64_t>(j->jend, a + chunk));
         rc = pfc::launch_walk(d.modes, (uint32_t)a, b, j->d_list, j->d_comp, j->d_clist, j->d_cprefix, d.nmin,
                               j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac, (uint32_t)j->fcap,
-                              cap, j->tuning.primes_per_lane, j->tuning.block_size, walk_bound(j, b), st);
+                              cap, j->tuning.primes_per_lane, j->tuning.block_size, walk_bound(j, b),
+                              j->tuning.walk_forms_off, st);
         if (rc) return rc;
     }
     e = hipEventRecord(j->evw1, st);
@@ -1166,7 +1170,8 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
                 (void)hipEventRecord(j->ev0, st);
                 rc = pfc::launch_walk(d.modes, j->jbegin, j->jbegin + steps, j->d_list, j->d_comp, j->d_clist,
                                       j->d_cprefix, d.nmin, j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl, j->d_fac,
-                                      (uint32_t)j->fcap, cap, ppl, blk, walk_bound(j, j->jbegin + steps), st);
+                                      (uint32_t)j->fcap, cap, ppl, blk, walk_bound(j, j->jbegin + steps),
+                                      j->tuning.walk_forms_off, st);
                 (void)hipEventRecord(j->ev1, st);
                 e = hipEventSynchronize(j->ev1);
                 if (e != hipSuccess) rc = hip_fail(e, "autotune walk");
@@ -1193,7 +1198,8 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
                 rc = pfc::launch_walk(d.modes, j->jbegin + a, j->jbegin + std::min(total, a + c), j->d_list,
                                       j->d_comp, j->d_clist, j->d_cprefix, d.nmin, j->d_p, j->d_v, j->d_ra, j->d_rb, j->d_ctrl,
                                       j->d_fac, (uint32_t)j->fcap, cap, win.primes_per_lane, win.block_size,
-                                      walk_bound(j, j->jbegin + std::min(total, a + c)), st);
+                                      walk_bound(j, j->jbegin + std::min(total, a + c)),
+                                      j->tuning.walk_forms_off, st);
             (void)hipEventRecord(j->ev1, st);
             e = hipEventSynchronize(j->ev1);
             if (e != hipSuccess && !rc) rc = hip_fail(e, "autotune walk chunk");
@@ -1270,17 +1276,17 @@ struct pfc_batcher {
     size_t d_jobs_cap = 0;
     pfc_batch_range* d_ranges = nullptr;
     size_t d_ranges_cap = 0;
-    uint32_t* d_words = nullptr;      // {queue head, range count, slot count}
+    uint32_t* d_words = nullptr;      // the round's device words (pfc::launch_batch), kBatchWords
     pfc_ctrl* h_ctrl = nullptr;       // pinned, one per slot
     size_t h_ctrl_cap = 0;
     uint64_t rounds = 0;
     // Captured device phase of a round (tuning.use_graph): uploads from the
-    // pinned staging below, queue-head clear, k_batch, control-block gather
-    // and one readback, replayed every round; captured again only when a
-    // buffer capacity or the launch shape changes.
+    // pinned staging below, queue-head clear, the set-up, walk and fold
+    // kernels, control-block gather and one readback, replayed every round;
+    // captured again only when a buffer capacity or the launch shape changes.
     pfc_batch_job* h_jobs = nullptr;  // pinned staging, d_jobs_cap entries
     pfc_batch_range* h_ranges = nullptr;  // pinned staging, d_ranges_cap entries
-    uint32_t* h_words = nullptr;      // pinned: {range count, slot count}
+    uint32_t* h_words = nullptr;      // pinned: d_words[6 .. 13) (the phases' range counts, the slot count)
     pfc_ctrl* d_ctrl_all = nullptr;   // gathered control blocks, d_jobs_cap entries
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1306,6 +1312,7 @@ pfc_batch_job batch_view(pfc_job* j) {
     b.ctrl = j->d_ctrl;
     b.fac = j->d_fac;
     b.fcap = (uint32_t)j->fcap;
+    b.forms_off = j->tuning.walk_forms_off;
     b.ta = pfc_setup_table{j->d_ca, j->d_ga, j->ta.npos, j->ta.digit_bits};
     b.tb = pfc_setup_table{j->d_cb, j->d_gb, j->tb.npos, j->tb.digit_bits};
     b.list = j->d_list;
@@ -1391,8 +1398,8 @@ extern "C" int pfc_batcher_create(pfc_device* dev, pfc_batcher** out) {
     pfc_batcher* b = new pfc_batcher();
     b->dev = dev;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->d_words, 4 * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_words, 4 * sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_words, pfc::kBatchWords * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_words, pfc::kBatchWords * sizeof(uint32_t), hipHostMallocDefault);
     if (e != hipSuccess) {
         int rc = hip_fail(e, "batcher create");
         if (b->d_words) (void)hipFree(b->d_words);
@@ -1429,7 +1436,7 @@ constexpr uint32_t kMinRoundSteps = 256;
 // block is gathered into h_ctrl with one readback.  Captured again only when
 // a buffer grows or the launch shape changes.
 static int batcher_graph_round(pfc_batcher* b, const std::vector<pfc_batch_job>& jobs,
-                               const std::vector<pfc_batch_range>& flat, uint32_t K) {
+                               const std::vector<pfc_batch_range>& flat, const uint32_t nphase[6], uint32_t K) {
     hipStream_t st = b->stream;
     const size_t ns = jobs.size();
     size_t hj_cap = b->graph_jobs_cap, hr_cap = b->graph_ranges_cap;
@@ -1468,13 +1475,17 @@ static int batcher_graph_round(pfc_batcher* b, const std::vector<pfc_batch_job>&
             e = hipMemcpyAsync(b->d_ranges, b->h_ranges, b->d_ranges_cap * sizeof(pfc_batch_range),
                                hipMemcpyHostToDevice, st);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(b->d_words + 1, b->h_words, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, st);
+            e = hipMemcpyAsync(b->d_words + 6, b->h_words, 7 * sizeof(uint32_t), hipMemcpyHostToDevice, st);
         if (e != hipSuccess) rc = hip_fail(e, "batcher graph upload");
-        if (!rc) rc = pfc::launch_zero_u32(b->d_words, st);
-        if (!rc)
-            rc = pfc::launch_batch(b->d_jobs, b->d_ranges, b->d_words + 1, b->d_words, K,
-                                   (uint32_t)b->dev->prop.multiProcessorCount * 32, st);
-        if (!rc) rc = pfc::launch_gather_ctrl(b->d_jobs, b->d_words + 2, b->d_ctrl_all, (uint32_t)b->d_jobs_cap, st);
+        if (!rc) rc = pfc::launch_zero_heads(b->d_words, st);
+        if (!rc) {
+            // a fixed grid per phase: the kernels read their range counts
+            // from d_words and return at once when a phase has none
+            const uint32_t nb = (uint32_t)b->dev->prop.multiProcessorCount * 32;
+            const uint32_t nblocks[6] = {nb, nb, nb, nb, nb, nb};
+            rc = pfc::launch_batch(b->d_jobs, b->d_ranges, b->d_words, K, nblocks, st);
+        }
+        if (!rc) rc = pfc::launch_gather_ctrl(b->d_jobs, b->d_words + 12, b->d_ctrl_all, (uint32_t)b->d_jobs_cap, st);
         if (!rc) {
             e = hipMemcpyAsync(b->h_ctrl, b->d_ctrl_all, b->d_jobs_cap * sizeof(pfc_ctrl), hipMemcpyDeviceToHost, st);
             if (e != hipSuccess) rc = hip_fail(e, "batcher graph readback");
@@ -1496,8 +1507,8 @@ static int batcher_graph_round(pfc_batcher* b, const std::vector<pfc_batch_job>&
     }
     memcpy(b->h_jobs, jobs.data(), ns * sizeof(pfc_batch_job));
     memcpy(b->h_ranges, flat.data(), flat.size() * sizeof(pfc_batch_range));
-    b->h_words[0] = (uint32_t)flat.size();
-    b->h_words[1] = (uint32_t)ns;
+    for (int ph = 0; ph < 6; ++ph) b->h_words[ph] = nphase[ph];
+    b->h_words[6] = (uint32_t)ns;
     hipError_t e = hipGraphLaunch(b->gexec, st);
     if (e != hipSuccess) return hip_fail(e, "batcher hipGraphLaunch");
     b->graph_replays += 1;
@@ -1571,7 +1582,7 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
     // ranges ordered by their items' cost, longest first, so that the queue
     // ends on short items.
     std::vector<pfc_batch_job> jobs(ns);
-    struct Tmp { pfc_batch_range r; uint32_t count; double cost; };
+    struct Tmp { pfc_batch_range r; uint32_t count; double cost, setup_cost; };
     std::vector<Tmp> ranges;
     uint32_t K;
     {
@@ -1597,47 +1608,70 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
         const uint32_t count = (uint32_t)((s.ncand + 64ull * K - 1) / (64ull * K));
         if (nitems + count > kMaxBatchItems) continue;   // left for the next round
         nitems += count;
-        // Relative cost of one item: walk steps (the F|C walk takes ~1.9 steps
-        // per n) plus the start-residue chunk products, each about two steps.
-        const double per = (double)(j1 - j0) * ((j->d_rb) ? 1.9 : 1.0) +
-                           ((flags & 1u) ? 2.0 * (double)(j->ta.chunks.size() + j->tb.chunks.size()) : 0.0);
-        ranges.push_back({{(uint32_t)i, j0, jw, flags, walk_bound(j, jw), 0, {0, 0}}, count, per});
+        // Relative cost of one walk item: walk steps (the F|C walk takes ~1.9
+        // steps per n).  The set-up list orders by start-table size.
+        const double per = (double)(j1 - j0) * ((j->d_rb) ? 1.9 : 1.0);
+        ranges.push_back({{(uint32_t)i, j0, jw, flags, walk_bound(j, jw), 0, {0, 0}}, count, per,
+                          (double)(j->ta.chunks.size() + j->tb.chunks.size())});
         s.jcur = j1;
         if (j1 == j->jend) finishing[i] = 1;
     }
     std::stable_sort(ranges.begin(), ranges.end(), [](const Tmp& a, const Tmp& c) { return a.cost > c.cost; });
     bool gathered = false;                 // every slot's control block is in h_ctrl after the sync
     if (!ranges.empty()) {
+        // Six lists in one array (pfc::launch_batch's phase order), each with
+        // its own item ends: the walk ranges of walk mode 0..3 (longest items
+        // first), the ranges of segments set up in this round (largest tables
+        // first), the ranges of segments folded in it.
+        std::vector<pfc_batch_range> flat;
+        flat.reserve(3 * ranges.size());
+        uint32_t nphase[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t nitem[6] = {0, 0, 0, 0, 0, 0};
+        auto append = [&](const Tmp& t, int ph) {
+            nitem[ph] += t.count;
+            flat.push_back(t.r);
+            flat.back().end = nitem[ph];
+            nphase[ph] += 1;
+        };
+        for (int mode = 0; mode < 4; ++mode)
+            for (const Tmp& t : ranges)
+                if (pfc::walk_mode_of(b->slots[t.r.job].job->desc.modes) == mode) append(t, mode);
+        std::vector<const Tmp*> setups;
+        for (const Tmp& t : ranges)
+            if (t.r.flags & 1u) setups.push_back(&t);
+        std::stable_sort(setups.begin(), setups.end(),
+                         [](const Tmp* a, const Tmp* c) { return a->setup_cost > c->setup_cost; });
+        for (const Tmp* t : setups) append(*t, 4);
+        for (const Tmp& t : ranges)
+            if (t.r.flags & 2u) append(t, 5);
         rc = ensure_device(&b->d_jobs, &b->d_jobs_cap, ns);
-        if (!rc) rc = ensure_device(&b->d_ranges, &b->d_ranges_cap, ranges.size());
+        if (!rc) rc = ensure_device(&b->d_ranges, &b->d_ranges_cap, flat.size());
         if (rc) return rc;
-        std::vector<pfc_batch_range> flat(ranges.size());
-        uint32_t end = 0;
-        for (size_t k = 0; k < ranges.size(); ++k) {
-            end += ranges[k].count;
-            flat[k] = ranges[k].r;
-            flat[k].end = end;
-        }
         bool use_graph;
         {
             std::lock_guard<std::mutex> g(b->dev->mu);
             use_graph = b->dev->tuning.use_graph != 0;
         }
         if (use_graph) {
-            rc = batcher_graph_round(b, jobs, flat, K);
+            rc = batcher_graph_round(b, jobs, flat, nphase, K);
             if (rc) return rc;
             gathered = true;
         } else {
-            b->h_words[0] = (uint32_t)flat.size();
+            for (int ph = 0; ph < 6; ++ph) b->h_words[ph] = nphase[ph];
+            b->h_words[6] = (uint32_t)ns;
             hipError_t e = hipMemcpyAsync(b->d_jobs, jobs.data(), ns * sizeof(pfc_batch_job), hipMemcpyHostToDevice, st);
             if (e == hipSuccess)
                 e = hipMemcpyAsync(b->d_ranges, flat.data(), flat.size() * sizeof(pfc_batch_range), hipMemcpyHostToDevice,
                                    st);
-            if (e == hipSuccess) e = hipMemcpyAsync(b->d_words + 1, b->h_words, sizeof(uint32_t), hipMemcpyHostToDevice, st);
-            if (e == hipSuccess) e = hipMemsetAsync(b->d_words, 0, sizeof(uint32_t), st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(b->d_words + 6, b->h_words, 7 * sizeof(uint32_t), hipMemcpyHostToDevice, st);
             if (e != hipSuccess) return hip_fail(e, "batcher upload");
-            const uint32_t nblocks = (uint32_t)std::min<uint64_t>(nitems, (uint64_t)b->dev->prop.multiProcessorCount * 32);
-            rc = pfc::launch_batch(b->d_jobs, b->d_ranges, b->d_words + 1, b->d_words, K, nblocks, st);
+            rc = pfc::launch_zero_heads(b->d_words, st);
+            if (rc) return rc;
+            const uint64_t nb = (uint64_t)b->dev->prop.multiProcessorCount * 32;
+            uint32_t nblocks[6];
+            for (int ph = 0; ph < 6; ++ph) nblocks[ph] = (uint32_t)std::min<uint64_t>(nitem[ph], nb);
+            rc = pfc::launch_batch(b->d_jobs, b->d_ranges, b->d_words, K, nblocks, st);
             if (rc) return rc;
         }
     }

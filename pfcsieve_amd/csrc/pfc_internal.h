@@ -19,7 +19,8 @@ struct pfc_batch_job {
     uint64_t* RB;
     pfc_ctrl* ctrl;
     pfc_factor* fac;
-    uint32_t fcap, pad;
+    uint32_t fcap;
+    uint32_t forms_off;        // pfc_tuning::walk_forms_off of the job
     pfc_setup_table ta;        // start table of res_a
     pfc_setup_table tb;        // start table of res_b (F|C)
     const uint32_t* list;
@@ -45,6 +46,15 @@ struct pfc_batch_range {
 
 namespace pfc {
 
+// Walk mode of a job's mode set, as the walk kernels are instantiated:
+// 0 = F, 1 = P, 2 = F|C, 3 = C; -1 for any other set.
+__host__ __device__ constexpr int walk_mode_of(uint32_t modes) {
+    return modes == PFC_MODE_FACTORIAL ? 0
+         : modes == PFC_MODE_PRIMORIAL ? 1
+         : modes == (PFC_MODE_FACTORIAL | PFC_MODE_COMPOSITORIAL) ? 2
+         : modes == PFC_MODE_COMPOSITORIAL ? 3 : -1;
+}
+
 extern thread_local char g_err[512];
 int set_error(int code, const char* msg);
 int hip_fail(hipError_t e, const char* what);
@@ -63,14 +73,22 @@ int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list
                 const uint32_t* d_comp, const uint32_t* d_clist, const uint32_t* d_cprefix, uint32_t nbase,
                 const uint64_t* d_p, const uint32_t* d_v, uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
                 pfc_factor* d_fac, uint32_t fcap, uint32_t cap, uint32_t ppl, uint32_t block,
-                uint32_t bmax, hipStream_t st);
+                uint32_t bmax, uint32_t forms_off, hipStream_t st);
 int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint32_t* d_v,
                  const uint64_t* d_one, const uint64_t* d_ra, const uint64_t* d_rb,
                  pfc_ctrl* d_ctrl, uint32_t cap, hipStream_t st);
-// d_nranges: device word holding the range count (read by the kernel).
-int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, const uint32_t* d_nranges,
-                 uint32_t* d_head, uint32_t ppl, uint32_t nblocks, hipStream_t st);
-int launch_zero_u32(uint32_t* d_p, hipStream_t st);
+// One batch round's kernels (k_batch phases): set-up (phase 4), the walks of
+// walk modes F, P, F|C and C (phases 0..3), fold (phase 5), in that order,
+// with nblocks[ph] workgroups for phase ph (0: not launched).  d_words: the
+// round's device words, read by the kernels: [0 .. 6) the phases' queue
+// heads, [6 .. 12) their range counts, [12] the slot count (for
+// launch_gather_ctrl).  d_ranges holds the phases' range lists in phase
+// order, each with its own item ends.
+constexpr uint32_t kBatchWords = 16;
+int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, uint32_t* d_words,
+                 uint32_t ppl, const uint32_t nblocks[6], hipStream_t st);
+// Clears the six queue heads of d_words.
+int launch_zero_heads(uint32_t* d_words, hipStream_t st);
 // d_out[i] = *d_jobs[i].ctrl for i < *d_nslots (cap: the most slots).
 int launch_gather_ctrl(const pfc_batch_job* d_jobs, const uint32_t* d_nslots, pfc_ctrl* d_out, uint32_t cap,
                        hipStream_t st);

@@ -362,6 +362,7 @@ struct WalkArgs {
     pfc_factor* fac;
     uint32_t fcap;
     uint32_t cap;
+    uint32_t forms_off;               // pfc_tuning::walk_forms_off
 };
 
 // One lane owns K candidates and keeps their residues in VGPRs for a whole
@@ -394,6 +395,15 @@ struct WalkArgs {
 // b * c < p / 2^14: the band B = bmax^2 << s on every third step finds the
 // hits of all three (the slow path also compares with +-(b * c) << s) and
 // passes a fraction below 2^-13 of the residues.
+//
+// Narrow form.  A wavefront whose every p lies in (2^34, 2^55) with
+// p * bmax < 2^61 (and bmax < 2^22) walks the launch on plain residues
+// X = x >> s in [0, 2p) with pfc::narrow_mul: five multiply instructions and
+// no correction per step, against ten instructions for the shifted step.
+// Groups and bands are the same, unshifted; a tested step first reduces its
+// result below p.  The launch converts its residues on entry and returns
+// them in canonical shifted form, so storage, set-up and the checksum fold
+// are unchanged and launches of one job may take either form.
 template <int K>
 struct Lanes {
     uint32_t k1[K], k0[K], v[K], s[K];    // k1, k0: pfc::walk_k1 / walk_k0 of d
@@ -432,6 +442,103 @@ __device__ __forceinline__ int group_steps(Lanes<K>& L, uint32_t bmax, bool trip
     for (int k = 0; k < K; ++k) {
         const uint64_t B = (g == 3 ? (uint64_t)bmax * bmax : (uint64_t)bmax) << L.s[k];
         pfc::walk_band(L.d[k], B, &L.c[k], &L.thr[k]);
+    }
+    return g;
+}
+
+// Narrow lanes: wavefronts whose every p and every multiplier of the launch
+// satisfy pfc::narrow_fits walk plain residues X in [0, 2p) with
+// pfc::narrow_mul (five multiplies and no correction per step) instead of
+// the shifted 3-by-2 step.  A tested step reduces its result to [0, p) with
+// one compare and an exec-masked add, then tests the band x <= B or
+// x >= p - B (one 64-bit add and compare), the same band as the shifted
+// walk's, unshifted.  The group sizes are those of the shifted walk (pairs,
+// and triples in P mode where p > 2^14 * bmax^2); p > 2^34 > 2 * bmax
+// always, so every test covers a pair at least.
+template <int K>
+struct NLanes {
+    uint64_t p[K], np[K];                 // np = 2^64 - p
+    uint64_t thr[K];                      // band threshold, pfc::narrow_band
+    uint64_t c;                           // band addend (the band is uniform)
+    uint32_t m[K], np0[K], kp1[K], s[K];  // pfc::narrow_mul constants; s = clz64(p)
+    bool live[K];
+};
+
+// True, uniformly, if every p of the wavefront fits the narrow walk for
+// multipliers up to bmax.
+template <int K>
+__device__ __forceinline__ bool narrow_wave(const Lanes<K>& L, uint32_t bmax) {
+    bool no = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) no |= !pfc::narrow_fits(L.d[k] >> L.s[k], bmax);
+    return __ballot(no) == 0;
+}
+
+// Steps per test (2, or 3 where triples are allowed) and the band constants.
+template <int K>
+__device__ __forceinline__ int narrow_lanes(NLanes<K>& N, const Lanes<K>& L, uint32_t bmax, bool triples) {
+    bool no3 = !triples || bmax >= (1u << 24);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t p = L.d[k] >> L.s[k];
+        N.p[k] = p;
+        N.np[k] = 0ull - p;
+        N.m[k] = pfc::narrow_m(L.v[k], L.s[k]);
+        N.np0[k] = pfc::narrow_np0(p);
+        N.kp1[k] = pfc::narrow_kp1(p);
+        N.s[k] = L.s[k];
+        N.live[k] = L.live[k];
+        no3 |= (p >> 14) <= (uint64_t)bmax * bmax;
+    }
+    const int g = __ballot(no3) != 0 ? 2 : 3;
+    const uint64_t B = g == 3 ? (uint64_t)bmax * bmax : (uint64_t)bmax;
+#pragma unroll
+    for (int k = 0; k < K; ++k) pfc::narrow_band(N.p[k], B, &N.c, &N.thr[k]);
+    return g;
+}
+
+// Medium lanes: wavefronts outside the narrow form whose every p lies in
+// (2^32, 2^63) and whose multipliers are below 2^28 walk residues at half
+// the shifted scale, x in [0, 2D) with D = p << (s - 1) = d / 2, with
+// pfc::medium_mul (seven instructions, no correction).  A tested step
+// reduces below D, then takes the shifted walk's high-word band test
+// against D.
+template <int K>
+struct MLanes {
+    uint64_t D[K], nD[K];                     // D = p << (s - 1); nD = 2^64 - D
+    uint32_t M[K], nd0[K], k1[K], s1[K];      // pfc::medium_mul constants; s1 = s - 1
+    uint32_t c[K], thr[K];                    // pfc::walk_band of D and the band
+    bool live[K];
+};
+
+template <int K>
+__device__ __forceinline__ bool medium_wave(const Lanes<K>& L, uint32_t bmax) {
+    bool no = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) no |= !pfc::medium_fits(L.d[k] >> L.s[k], bmax);
+    return __ballot(no) == 0;
+}
+
+template <int K>
+__device__ __forceinline__ int medium_lanes(MLanes<K>& Q, const Lanes<K>& L, uint32_t bmax, bool triples) {
+    bool no3 = !triples || bmax >= (1u << 24);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t D = L.d[k] >> 1;
+        Q.D[k] = D;
+        Q.nD[k] = 0ull - D;
+        Q.M[k] = pfc::medium_m(L.v[k]);
+        Q.nd0[k] = pfc::walk_k0((uint32_t)D);
+        Q.k1[k] = pfc::walk_k1((uint32_t)(D >> 32), (uint32_t)D);
+        Q.s1[k] = L.s[k] - 1;
+        Q.live[k] = L.live[k];
+        no3 |= ((L.d[k] >> L.s[k]) >> 14) <= (uint64_t)bmax * bmax;
+    }
+    const int g = __ballot(no3) != 0 ? 2 : 3;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t B = (g == 3 ? (uint64_t)bmax * bmax : (uint64_t)bmax) << Q.s1[k];
+        pfc::walk_band(Q.D[k], B, &Q.c[k], &Q.thr[k]);
     }
     return g;
 }
@@ -521,6 +628,135 @@ __device__ __forceinline__ void settle(uint64_t (&r)[K], uint32_t m2, uint32_t m
     }
 }
 
+// The narrow forms of the primitives above.  An untested step leaves r in
+// [0, 2p); a tested step reduces it to [0, p) before the band test.
+template <int K>
+__device__ __forceinline__ void mul_step(uint64_t (&r)[K], uint32_t n, const NLanes<K>& N) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) r[k] = pfc::narrow_mul(r[k], n, N.m[k], N.np0[k], N.kp1[k]);
+}
+
+// r[k] from [0, 2p) to [0, p): an exec-masked add of 2^64 - p, written as
+// the instruction (one 64-bit add) so that the optimiser neither turns the
+// branch into selects nor the add into a subtract-with-borrow pair.
+template <int K>
+__device__ __forceinline__ void narrow_reduce(uint64_t (&r)[K], const NLanes<K>& N) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (r[k] >= N.p[k]) {
+            uint64_t y;
+            asm volatile("v_lshl_add_u64 %0, %1, 0, %2" : "=v"(y) : "v"(r[k]), "v"(N.np[k]));
+            r[k] = y;
+        }
+    }
+}
+
+template <int K>
+__device__ __forceinline__ bool attention(const uint64_t (&r)[K], const NLanes<K>& N) {
+    uint64_t rare = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) rare |= __ballot(pfc::narrow_attention(r[k], N.c, N.thr[k]));
+    return __builtin_expect(rare != 0, 0);
+}
+
+template <int K>
+__device__ __forceinline__ bool step(uint64_t (&r)[K], uint32_t n, const NLanes<K>& N) {
+    mul_step<K>(r, n, N);
+    narrow_reduce<K>(r, N);
+    return attention<K>(r, N);
+}
+
+// Slow path of a narrow tested step: r reduced (a last untested composite
+// step also arrives here, in [0, 2p)), then the hits recorded as in the
+// shifted walk (record_lane takes the shifted forms X << s, p << s).
+template <int K>
+__device__ __forceinline__ void settle(uint64_t (&r)[K], uint32_t m2, uint32_t m, uint32_t n,
+                                       int32_t type, const NLanes<K>& N, pfc_factor* fac, uint32_t fcap,
+                                       pfc_ctrl* ctrl) {
+    narrow_reduce<K>(r, N);
+    uint64_t hits = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t p = N.p[k];
+        hits |= __ballot(r[k] == 1) | __ballot(r[k] == p - 1);
+        if (m) hits |= __ballot(r[k] == n) | __ballot(r[k] == p - n);
+        if (m2) {
+            const uint64_t b = (uint64_t)m * n;
+            hits |= __ballot(r[k] == b) | __ballot(r[k] == p - b);
+        }
+    }
+    if (hits) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            record_lane(r[k] << N.s[k], N.p[k] << N.s[k], N.s[k], N.live[k], m2, m, n, type, fac, fcap, ctrl);
+    }
+}
+
+// The medium forms.  An untested step leaves r in [0, 2D); a tested step
+// reduces it below D before the high-word band test.
+template <int K>
+__device__ __forceinline__ void mul_step(uint64_t (&r)[K], uint32_t n, const MLanes<K>& Q) {
+    const uint32_t n4 = n << 2;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r[k] = pfc::medium_mul(r[k], n, n4, Q.M[k], Q.nd0[k], Q.k1[k]);
+}
+
+// r[k] from [0, 2D) to [0, D), as narrow_reduce.
+template <int K>
+__device__ __forceinline__ void medium_reduce(uint64_t (&r)[K], const MLanes<K>& Q) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (r[k] >= Q.D[k]) {
+            uint64_t y;
+            asm volatile("v_lshl_add_u64 %0, %1, 0, %2" : "=v"(y) : "v"(r[k]), "v"(Q.nD[k]));
+            r[k] = y;
+        }
+    }
+}
+
+template <int K>
+__device__ __forceinline__ bool attention(const uint64_t (&r)[K], const MLanes<K>& Q) {
+    uint64_t rare = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) rare |= __ballot(pfc::walk_attention(r[k], Q.c[k], Q.thr[k]));
+    return __builtin_expect(rare != 0, 0);
+}
+
+template <int K>
+__device__ __forceinline__ bool step(uint64_t (&r)[K], uint32_t n, const MLanes<K>& Q) {
+    mul_step<K>(r, n, Q);
+    medium_reduce<K>(r, Q);
+    return attention<K>(r, Q);
+}
+
+// Slow path: r reduced below D (a last untested composite step arrives in
+// [0, 2D)), then the shifted walk's comparisons at scale s - 1.
+template <int K>
+__device__ __forceinline__ void settle(uint64_t (&r)[K], uint32_t m2, uint32_t m, uint32_t n,
+                                       int32_t type, const MLanes<K>& Q, pfc_factor* fac, uint32_t fcap,
+                                       pfc_ctrl* ctrl) {
+    medium_reduce<K>(r, Q);
+    uint64_t hits = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t one = 1ull << Q.s1[k];
+        hits |= __ballot(r[k] == one) | __ballot(r[k] == Q.D[k] - one);
+        if (m) {
+            const uint64_t b = (uint64_t)n << Q.s1[k];
+            hits |= __ballot(r[k] == b) | __ballot(r[k] == Q.D[k] - b);
+        }
+        if (m2) {
+            const uint64_t b = ((uint64_t)m * n) << Q.s1[k];
+            hits |= __ballot(r[k] == b) | __ballot(r[k] == Q.D[k] - b);
+        }
+    }
+    if (hits) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            record_lane(r[k], Q.D[k], Q.s1[k], Q.live[k], m2, m, n, type, fac, fcap, ctrl);
+    }
+}
+
 // Multiplier applications performed by a wavefront, counted in a scalar
 // register where the steps are taken (walk-completion evidence, summed into
 // pfc_ctrl::walked).  The empty asm after each increment makes the count
@@ -547,17 +783,17 @@ struct WalkStream {
 };
 
 // One tested step of a chain, alone (no untested step before it).
-template <int K>
-__device__ __forceinline__ void single(uint64_t (&r)[K], uint32_t n, int32_t type, const Lanes<K>& L,
+template <int K, class LT>
+__device__ __forceinline__ void single(uint64_t (&r)[K], uint32_t n, int32_t type, const LT& L,
                                        const WalkStream& W, Tally& T) {
     tally(T, 1);
     if (step<K>(r, n, L)) settle<K>(r, 0, 0, n, type, L, W.fac, W.fcap, W.ctrl);
 }
 
 // Steps m (untested) and n (tested) of a chain.
-template <int K>
+template <int K, class LT>
 __device__ __forceinline__ void pair(uint64_t (&r)[K], uint32_t m, uint32_t n, int32_t type,
-                                     const Lanes<K>& L, const WalkStream& W, Tally& T) {
+                                     const LT& L, const WalkStream& W, Tally& T) {
     tally(T, 2);
     mul_step<K>(r, m, L);
     untested<K>(r);
@@ -568,11 +804,31 @@ __device__ __forceinline__ void pair(uint64_t (&r)[K], uint32_t m, uint32_t n, i
 // input (a tested step's result, so below d) and m * n is below d * 2^32, as
 // the 3-by-2 remainder needs, and the result and its test are those of
 // pair(): a hit at m leaves +-n << s, one at n +-1 << s.
-template <int K>
+template <int K, class LT>
 __device__ __forceinline__ void pair_product(uint64_t (&r)[K], uint32_t m, uint32_t n, int32_t type,
-                                             const Lanes<K>& L, const WalkStream& W, Tally& T) {
+                                             const LT& L, const WalkStream& W, Tally& T) {
     tally(T, 2);
     if (step<K>(r, m * n, L)) settle<K>(r, 0, m, n, type, L, W.fac, W.fcap, W.ctrl);
+}
+
+// The narrow walk has no product step (X * m * n may exceed 2^64): two steps.
+template <int K>
+__device__ __forceinline__ void pair_product(uint64_t (&r)[K], uint32_t m, uint32_t n, int32_t type,
+                                             const NLanes<K>& L, const WalkStream& W, Tally& T) {
+    pair<K>(r, m, n, type, L, W, T);
+}
+
+// The medium walk takes a pair as one step by m * n while m * n < 2^28 (its
+// multiplier bound; any input below 2D is accepted), else two steps.
+template <int K>
+__device__ __forceinline__ void pair_product(uint64_t (&r)[K], uint32_t m, uint32_t n, int32_t type,
+                                             const MLanes<K>& L, const WalkStream& W, Tally& T) {
+    if (m * n < (1u << 28)) {
+        tally(T, 2);
+        if (step<K>(r, m * n, L)) settle<K>(r, 0, m, n, type, L, W.fac, W.fcap, W.ctrl);
+    } else {
+        pair<K>(r, m, n, type, L, W, T);
+    }
 }
 
 // Pairs of multipliers m < n <= kProductLimit have m * n < 2^32
@@ -580,9 +836,9 @@ __device__ __forceinline__ void pair_product(uint64_t (&r)[K], uint32_t m, uint3
 constexpr uint32_t kProductLimit = 65536;
 
 // Steps m2, m (untested) and n (tested) of a chain.
-template <int K>
+template <int K, class LT>
 __device__ __forceinline__ void triple(uint64_t (&r)[K], uint32_t m2, uint32_t m, uint32_t n,
-                                       int32_t type, const Lanes<K>& L, const WalkStream& W, Tally& T) {
+                                       int32_t type, const LT& L, const WalkStream& W, Tally& T) {
     tally(T, 3);
     mul_step<K>(r, m2, L);
     untested<K>(r);
@@ -594,8 +850,8 @@ __device__ __forceinline__ void triple(uint64_t (&r)[K], uint32_t m2, uint32_t m
 // Factorial steps n = j for j in [j0, j1).  Four steps per trip: one scalar
 // compare-and-branch per four updates.  Trips whose last multiplier is at
 // most kProductLimit take each pair as one step by its product.
-template <int K, int G>
-__device__ __forceinline__ void factorial_steps(uint64_t (&r)[K], const Lanes<K>& L, uint32_t j0,
+template <int K, int G, class LT>
+__device__ __forceinline__ void factorial_steps(uint64_t (&r)[K], const LT& L, uint32_t j0,
                                                 uint32_t j1, const WalkStream& W, Tally& T) {
     constexpr bool PAIRED = G >= 2;
     uint32_t j = j0;
@@ -629,9 +885,9 @@ __device__ __forceinline__ void factorial_steps(uint64_t (&r)[K], const Lanes<K>
 // load per 16 steps keeps the scalar pipe free for the step bookkeeping.  A
 // block whose last (largest) multiplier is at most kProductLimit takes its
 // pairs as product steps.
-template <int K, int G>
+template <int K, int G, class LT>
 __device__ __forceinline__ void list_steps(uint64_t (&r)[K], const uint32_t* list, uint32_t j0, uint32_t j1,
-                                           int32_t type, const Lanes<K>& L, const WalkStream& W, Tally& T) {
+                                           int32_t type, const LT& L, const WalkStream& W, Tally& T) {
     constexpr bool PAIRED = G >= 2;
     const ConstU32 lst = (ConstU32)list;
     uint32_t j = j0;
@@ -705,8 +961,8 @@ __device__ __forceinline__ uint32_t comp_index(const WalkStream& W, uint32_t n) 
 }
 
 // G: steps per test (group_steps); 3 is used by P mode only.
-template <int MODE, int K, int G>
-__device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K], const Lanes<K>& L,
+template <int MODE, int K, int G, class LT>
+__device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K], const LT& L,
                                            uint32_t j0, uint32_t j1, const WalkStream& W, Tally& T) {
     constexpr bool PAIRED = G >= 2;
     if (MODE == 0) {
@@ -811,8 +1067,51 @@ __device__ __forceinline__ void walk_steps(uint64_t (&ra)[K], uint64_t (&rb)[K],
 // Returns the multiplier applications the wavefront performed.
 template <int MODE, int K>
 __device__ __forceinline__ uint32_t walk_range(uint64_t (&ra)[K], uint64_t (&rb)[K], Lanes<K>& L,
-                                               uint32_t j0, uint32_t j1, uint32_t bmax, const WalkStream& W) {
+                                               uint32_t j0, uint32_t j1, uint32_t bmax, uint32_t forms_off,
+                                               const WalkStream& W) {
     Tally T{0};
+    if (!(forms_off & 1u) && narrow_wave<K>(L, bmax)) {
+        // Plain residues for the launch: (x >> s) < 2p also for an unreduced
+        // shifted x < 2^64 < 2d; back to canonical shifted form at the end.
+        NLanes<K> N;
+        const int gn = narrow_lanes<K>(N, L, bmax, MODE == 1);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            ra[k] >>= L.s[k];
+            if (MODE == 2) rb[k] >>= L.s[k];
+        }
+        if (MODE == 1 && gn == 3) walk_steps<MODE, K, 3>(ra, rb, N, j0, j1, W, T);
+        else walk_steps<MODE, K, 2>(ra, rb, N, j0, j1, W, T);
+        narrow_reduce<K>(ra, N);
+        if (MODE == 2) narrow_reduce<K>(rb, N);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            ra[k] <<= L.s[k];
+            if (MODE == 2) rb[k] <<= L.s[k];
+        }
+        return T.n;
+    }
+    if (!(forms_off & 2u) && medium_wave<K>(L, bmax)) {
+        // Half the shifted scale: x >> 1 < 2^63 < 2D also for an unreduced
+        // shifted x; back to canonical shifted form at the end.
+        MLanes<K> Q;
+        const int gm = medium_lanes<K>(Q, L, bmax, MODE == 1);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            ra[k] >>= 1;
+            if (MODE == 2) rb[k] >>= 1;
+        }
+        if (MODE == 1 && gm == 3) walk_steps<MODE, K, 3>(ra, rb, Q, j0, j1, W, T);
+        else walk_steps<MODE, K, 2>(ra, rb, Q, j0, j1, W, T);
+        medium_reduce<K>(ra, Q);
+        if (MODE == 2) medium_reduce<K>(rb, Q);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            ra[k] <<= 1;
+            if (MODE == 2) rb[k] <<= 1;
+        }
+        return T.n;
+    }
     const int g = group_steps<K>(L, bmax, MODE == 1);
     if constexpr (MODE == 1) {
         if (g == 3) {
@@ -837,12 +1136,6 @@ __device__ __forceinline__ void add_walked(uint32_t taken, const Lanes<K>& L, pf
         atomicAdd((unsigned long long*)&ctrl->walked, (unsigned long long)taken * live);
 }
 
-__host__ __device__ constexpr int walk_mode(uint32_t modes) {
-    return modes == PFC_MODE_FACTORIAL ? 0
-         : modes == PFC_MODE_PRIMORIAL ? 1
-         : modes == (PFC_MODE_FACTORIAL | PFC_MODE_COMPOSITORIAL) ? 2
-         : modes == PFC_MODE_COMPOSITORIAL ? 3 : -1;
-}
 
 template <int MODE, int K>
 __global__ __launch_bounds__(256) void k_walk(const WalkArgs A) {
@@ -864,7 +1157,7 @@ __global__ __launch_bounds__(256) void k_walk(const WalkArgs A) {
         rb[k] = (MODE == 2) ? A.RB[src] : 0;
     }
     const WalkStream W{A.list, A.composite, A.clist, A.cprefix, A.nbase, A.fac, A.fcap, A.ctrl};
-    const uint32_t taken = walk_range<MODE, K>(ra, rb, L, A.j0, A.j1, A.bmax, W);
+    const uint32_t taken = walk_range<MODE, K>(ra, rb, L, A.j0, A.j1, A.bmax, A.forms_off, W);
     add_walked<K>(taken, L, A.ctrl);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -927,90 +1220,115 @@ __global__ __launch_bounds__(256) void k_check(uint32_t modes, uint32_t nmax,
 
 // ------------------------------------------------------------------ batch
 
-// Persistent fused kernel over a device work queue.  One workgroup is one
-// wavefront; it takes items (job, 64*K candidates, step range) from the
-// queue until the queue is drained.  An item with kBatchSetup first builds
-// the candidates' reciprocals and start residues (as k_setup); an item with
-// kBatchCheck finally folds their checksum terms (as k_check).  Items of one
-// launch may come from different jobs with different modes, tables and
-// n-ranges: each wavefront's stream is uniform, the launch is not.
+// Persistent kernels over device work queues, six phases per round: set-up of
+// new segments, the walk of the segments of each walk mode (F, P, F|C, C),
+// and the checksum fold of finished segments.  One workgroup is one
+// wavefront; it takes items (job, 64*K candidates) from its phase's queue
+// until the queue is drained.  Items of one launch may come from different
+// jobs with different tables, n-ranges and moduli: each wavefront's stream is
+// uniform, the launch is not.  The phases are separate kernels so that each
+// walk compiles as lean as k_walk<MODE> (one mode's code, no set-up or fold
+// state live across it, the same register budget and occupancy).
 constexpr uint32_t kBatchSetup = 1u, kBatchCheck = 2u;
+constexpr int kPhaseSetup = 4, kPhaseFold = 5, kPhases = 6;   // phases 0..3: the walk of mode 0..3
 
+// Reciprocal, R mod p and start residues of the candidates of an item (as
+// k_setup).
 template <int K>
-__device__ __forceinline__ void batch_item(const pfc_batch_job& J, const pfc_batch_item& it) {
+__device__ __forceinline__ void batch_setup_item(const pfc_batch_job& J, const pfc_batch_item& it) {
+    uint32_t n_c = J.ctrl->ncand;
+    if (n_c > J.cap) n_c = J.cap;
+    const uint32_t first = it.first + __lane_id() * K;
+    const bool two = J.modes == (PFC_MODE_FACTORIAL | PFC_MODE_COMPOSITORIAL);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t idx = first + k;
+        if (idx >= n_c) continue;
+        const Modulus m = pfc::make_modulus(J.P[idx]);
+        J.V[idx] = m.v;
+        J.ONE[idx] = pfc::mont_one(m);
+        J.RA[idx] = horner_start(J.ta, m);
+        if (two) J.RB[idx] = horner_start(J.tb, m);
+    }
+}
+
+// The walk of an item's candidates over its multiplier steps (walk mode MODE).
+template <int K, int MODE>
+__device__ __forceinline__ void batch_walk_item(const pfc_batch_job& J, const pfc_batch_item& it) {
     uint32_t n_c = J.ctrl->ncand;
     if (n_c > J.cap) n_c = J.cap;
     const uint32_t first = it.first + __lane_id() * K;
     Lanes<K> L;
-    uint64_t ra[K], rb[K], mone[K];
+    uint64_t ra[K], rb[K];
     const bool two = J.modes == (PFC_MODE_FACTORIAL | PFC_MODE_COMPOSITORIAL);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const uint32_t idx = first + k;
         L.live[k] = idx < n_c;
         const uint32_t src = L.live[k] ? idx : it.first;   // items never start past ncand
-        const uint64_t p = J.P[src];
-        if (it.flags & kBatchSetup) {
-            const Modulus m = pfc::make_modulus(p);
-            mone[k] = pfc::mont_one(m);
-            lanes_from_state<K>(L, p, m.v, k);
-            ra[k] = horner_start(J.ta, m);
-            rb[k] = two ? horner_start(J.tb, m) : 0;
-        } else {
-            mone[k] = 0;
-            lanes_from_state<K>(L, p, J.V[src], k);
-            ra[k] = J.RA[src];
-            rb[k] = two ? J.RB[src] : 0;
-        }
+        lanes_from_state<K>(L, J.P[src], J.V[src], k);
+        ra[k] = J.RA[src];
+        rb[k] = two ? J.RB[src] : 0;
     }
     const WalkStream W{J.list, J.composite, J.clist, J.cprefix, J.nmin, J.fac, J.fcap, J.ctrl};
-    uint32_t taken;
-    switch (walk_mode(J.modes)) {
-        case 0: taken = walk_range<0, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
-        case 1: taken = walk_range<1, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
-        case 2: taken = walk_range<2, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
-        default: taken = walk_range<3, K>(ra, rb, L, it.j0, it.j1, it.bmax, W); break;
-    }
+    const uint32_t taken = walk_range<MODE, K>(ra, rb, L, it.j0, it.j1, it.bmax, J.forms_off, W);
     add_walked<K>(taken, L, J.ctrl);
-    uint64_t sum = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (!L.live[k]) continue;
-        const uint32_t idx = first + k;
-        J.RA[idx] = ra[k];
-        if (two) J.RB[idx] = rb[k];
-        if (it.flags & kBatchSetup) {
-            J.V[idx] = L.v[k];
-            J.ONE[idx] = mone[k];
-        }
-        if (it.flags & kBatchCheck) {
-            Modulus m;
-            m.d1 = (uint32_t)(L.d[k] >> 32);
-            m.d0 = (uint32_t)L.d[k];
-            m.v = L.v[k];
-            m.s = L.s[k];
-            sum += check_term(J.modes, J.nmax, m, (it.flags & kBatchSetup) ? mone[k] : J.ONE[idx],
-                              ra[k], rb[k]);
-        }
-    }
-    if (it.flags & kBatchCheck) {
-        wave_add_u64(sum, &J.ctrl->checksum);
-        if (it.first == 0 && __lane_id() == 0)
-            atomicAdd((unsigned long long*)&J.ctrl->primecount, (unsigned long long)n_c);
+        J.RA[first + k] = ra[k];
+        if (two) J.RB[first + k] = rb[k];
     }
 }
 
-// The queue is a counter over the launch's items; an item's job segment is
-// found by a wavefront-uniform binary search over the ranges' ends.  The
-// range count is read from device memory, so that one captured launch
-// serves every round of a batcher (the host writes the count before each
-// replay); a launch with no ranges returns at once.
+// Checksum terms of an item's candidates (as k_check, with the same check of
+// every stored reciprocal); the item at candidate 0 also counts the segment's
+// candidates.
 template <int K>
+__device__ __forceinline__ void batch_fold_item(const pfc_batch_job& J, const pfc_batch_item& it) {
+    uint32_t n_c = J.ctrl->ncand;
+    if (n_c > J.cap) n_c = J.cap;
+    const uint32_t first = it.first + __lane_id() * K;
+    const bool two = J.modes == (PFC_MODE_FACTORIAL | PFC_MODE_COMPOSITORIAL);
+    uint64_t sum = 0;
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t idx = first + k;
+        if (idx >= n_c) continue;
+        const uint64_t p = J.P[idx];
+        Modulus m;
+        m.s = pfc::clz64(p);
+        const uint64_t d = p << m.s;
+        m.d1 = (uint32_t)(d >> 32);
+        m.d0 = (uint32_t)d;
+        m.v = J.V[idx];
+        bad |= !pfc::modulus_ok(p, m);
+        sum += check_term(J.modes, J.nmax, m, J.ONE[idx], J.RA[idx], two ? J.RB[idx] : 0);
+    }
+    if (__ballot(bad) != 0 && __lane_id() == 0) atomicOr(&J.ctrl->flags, 2u);
+    wave_add_u64(sum, &J.ctrl->checksum);
+    if (it.first == 0 && __lane_id() == 0)
+        atomicAdd((unsigned long long*)&J.ctrl->primecount, (unsigned long long)n_c);
+}
+
+// The queue of a phase is a counter over its items; an item's job segment is
+// found by a wavefront-uniform binary search over the ranges' ends.  The
+// phases' ranges lie in one array in phase order, each list with its own item
+// ends; words[kPhases + ph] is phase ph's range count and words[ph] its queue
+// head.  Counts are read from device memory, so that one captured round
+// serves every round of a batcher (the host writes them before each replay);
+// a phase with no ranges returns at once.
+template <int K, int PHASE>
 __global__ __launch_bounds__(64) void k_batch(const pfc_batch_job* __restrict__ jobs,
-                                              const pfc_batch_range* __restrict__ ranges,
-                                              const uint32_t* __restrict__ nranges_p, uint32_t* head) {
-    const uint32_t nranges = *nranges_p;
+                                              const pfc_batch_range* __restrict__ all_ranges, uint32_t* words) {
+    const uint32_t nranges = words[kPhases + PHASE];
     if (nranges == 0) return;
+    uint32_t base = 0;
+#pragma unroll
+    for (int ph = 0; ph < PHASE; ++ph) base += words[kPhases + ph];
+    const pfc_batch_range* ranges = all_ranges + base;
+    uint32_t* head = words + PHASE;
     const uint32_t nitems = ranges[nranges - 1].end;
     for (;;) {
         uint32_t i = 0;
@@ -1033,13 +1351,16 @@ __global__ __launch_bounds__(64) void k_batch(const pfc_batch_job* __restrict__ 
         it.flags = R.flags;
         it.bmax = R.bmax;
         const pfc_batch_job J = jobs[R.job];
-        batch_item<K>(J, it);
+        if constexpr (PHASE < 4) batch_walk_item<K, PHASE>(J, it);
+        else if constexpr (PHASE == kPhaseSetup) batch_setup_item<K>(J, it);
+        else batch_fold_item<K>(J, it);
     }
 }
 
-// Clears the batch queue head (a kernel, not a memset node, in captured graphs).
-__global__ void k_zero_u32(uint32_t* p) {
-    if (threadIdx.x == 0) *p = 0;
+// Clears the queue heads of a batch round (words[0 .. kPhases); a kernel, not
+// a memset node, in captured graphs).
+__global__ void k_zero_heads(uint32_t* words) {
+    if (threadIdx.x < kPhases) words[threadIdx.x] = 0;
 }
 
 // Copies every slot's control block into one array, for one readback per
@@ -1186,7 +1507,7 @@ int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list
                 const uint32_t* d_comp, const uint32_t* d_clist, const uint32_t* d_cprefix, uint32_t nbase,
                 const uint64_t* d_p, const uint32_t* d_v, uint64_t* d_ra, uint64_t* d_rb, pfc_ctrl* d_ctrl,
                 pfc_factor* d_fac, uint32_t fcap, uint32_t cap, uint32_t ppl, uint32_t block,
-                uint32_t bmax, hipStream_t st) {
+                uint32_t bmax, uint32_t forms_off, hipStream_t st) {
     if (j1 <= j0 || cap == 0) return PFC_OK;
     if ((d_clist == nullptr) != (d_cprefix == nullptr))
         return set_error(PFC_ERR_INVALID, "walk: composite list without its prefix counts");
@@ -1194,7 +1515,7 @@ int launch_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list
     A.j0 = j0; A.j1 = j1; A.bmax = bmax; A.list = d_list; A.composite = d_comp; A.nbase = nbase;
     A.clist = d_clist; A.cprefix = d_cprefix;
     A.P = d_p; A.V = d_v; A.RA = d_ra; A.RB = d_rb; A.ctrl = d_ctrl;
-    A.fac = d_fac; A.fcap = fcap; A.cap = cap;
+    A.fac = d_fac; A.fcap = fcap; A.cap = cap; A.forms_off = forms_off;
     if (block != 64 && block != 128 && block != 256) block = 256;
     int mode;
     if (modes == PFC_MODE_FACTORIAL) {
@@ -1236,21 +1557,31 @@ int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint3
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_check launch");
 }
 
-int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, const uint32_t* d_nranges,
-                 uint32_t* d_head, uint32_t ppl, uint32_t nblocks, hipStream_t st) {
-    if (nblocks == 0) nblocks = 1;
-    if (ppl == 1) hipLaunchKernelGGL(k_batch<1>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_ranges, d_nranges, d_head);
-    else if (ppl == 2) hipLaunchKernelGGL(k_batch<2>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_ranges, d_nranges, d_head);
-    else if (ppl == 4) hipLaunchKernelGGL(k_batch<4>, dim3(nblocks), dim3(64), 0, st, d_jobs, d_ranges, d_nranges, d_head);
+template <int K>
+static void batch_phases(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, uint32_t* d_words,
+                         const uint32_t nblocks[kPhases], hipStream_t st) {
+#define PFC_BATCH_PHASE(PH) \
+    if (nblocks[PH]) hipLaunchKernelGGL((k_batch<K, PH>), dim3(nblocks[PH]), dim3(64), 0, st, d_jobs, d_ranges, d_words);
+    PFC_BATCH_PHASE(4)
+    PFC_BATCH_PHASE(0) PFC_BATCH_PHASE(1) PFC_BATCH_PHASE(2) PFC_BATCH_PHASE(3)
+    PFC_BATCH_PHASE(5)
+#undef PFC_BATCH_PHASE
+}
+
+int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, uint32_t* d_words,
+                 uint32_t ppl, const uint32_t nblocks[6], hipStream_t st) {
+    if (ppl == 1) batch_phases<1>(d_jobs, d_ranges, d_words, nblocks, st);
+    else if (ppl == 2) batch_phases<2>(d_jobs, d_ranges, d_words, nblocks, st);
+    else if (ppl == 4) batch_phases<4>(d_jobs, d_ranges, d_words, nblocks, st);
     else return set_error(PFC_ERR_INVALID, "batch: primes_per_lane must be 1, 2 or 4");
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PFC_OK : hip_fail(e, "k_batch launch");
 }
 
-int launch_zero_u32(uint32_t* d_p, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, d_p);
+int launch_zero_heads(uint32_t* d_words, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_heads, dim3(1), dim3(64), 0, st, d_words);
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? PFC_OK : hip_fail(e, "k_zero_u32 launch");
+    return e == hipSuccess ? PFC_OK : hip_fail(e, "k_zero_heads launch");
 }
 
 int launch_gather_ctrl(const pfc_batch_job* d_jobs, const uint32_t* d_nslots, pfc_ctrl* d_out, uint32_t cap,
@@ -1387,7 +1718,7 @@ int pfc_stage_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_l
     // cannot bound without reading it: P steps are then each tested alone.
     const uint32_t bmax = (modes == PFC_MODE_PRIMORIAL || j1 <= j0) ? 0 : j1 - 1;
     return pfc::launch_walk(modes, j0, j1, d_list, d_composite, nullptr, nullptr, nbase, d_p, d_v, d_res_a,
-                            d_res_b, d_ctrl, d_fac, fcap, cap, 4, 256, bmax, (hipStream_t)stream);
+                            d_res_b, d_ctrl, d_fac, fcap, cap, 4, 256, bmax, 0, (hipStream_t)stream);
 }
 
 int pfc_stage_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint32_t* d_v,

@@ -80,6 +80,46 @@ int t_walk_attention(uint64_t p, uint64_t B, uint64_t x) {
     pfc::walk_band(pfc::mod_d(m), B, &c, &thr);
     return pfc::walk_attention(x, c, thr);
 }
+// The narrow walk (pfc::narrow_mul): its constants from the modulus as the
+// kernels derive them; out[0] = m, out[1] = np0, out[2] = kp1.
+void t_narrow_consts(uint64_t p, uint32_t out[3]) {
+    const pfc::Modulus m = pfc::make_modulus(p);
+    out[0] = pfc::narrow_m(m.v, m.s);
+    out[1] = pfc::narrow_np0(p);
+    out[2] = pfc::narrow_kp1(p);
+}
+int t_narrow_fits(uint64_t p, uint32_t bmax) { return pfc::narrow_fits(p, bmax); }
+// `count` narrow steps from x by the multipliers n[0 .. count); returns the
+// unreduced result (in [0, 2p) when the step's bounds hold).
+uint64_t t_narrow_walk(uint64_t p, uint64_t x, const uint32_t* n, uint32_t count) {
+    const pfc::Modulus m = pfc::make_modulus(p);
+    const uint32_t mm = pfc::narrow_m(m.v, m.s), np0 = pfc::narrow_np0(p), kp1 = pfc::narrow_kp1(p);
+    for (uint32_t i = 0; i < count; ++i) x = pfc::narrow_mul(x, n[i], mm, np0, kp1);
+    return x;
+}
+// The narrow walk's band test of a reduced x against the band B.
+int t_narrow_attention(uint64_t p, uint64_t B, uint64_t x) {
+    uint64_t c, thr;
+    pfc::narrow_band(p, B, &c, &thr);
+    return pfc::narrow_attention(x, c, thr);
+}
+// The medium walk (pfc::medium_mul) at half the shifted scale, D = p << (s - 1):
+// `count` steps from x (a multiple of 2^(s-1) below 2D) by n[0 .. count);
+// returns the unreduced result.  out[0] = M, out[1] = nd0, out[2] = k1.
+void t_medium_consts(uint64_t p, uint32_t out[3]) {
+    const pfc::Modulus m = pfc::make_modulus(p);
+    const uint64_t D = p << (m.s - 1);
+    out[0] = pfc::medium_m(m.v);
+    out[1] = pfc::walk_k0((uint32_t)D);
+    out[2] = pfc::walk_k1((uint32_t)(D >> 32), (uint32_t)D);
+}
+int t_medium_fits(uint64_t p, uint32_t bmax) { return pfc::medium_fits(p, bmax); }
+uint64_t t_medium_walk(uint64_t p, uint64_t x, const uint32_t* n, uint32_t count) {
+    uint32_t c[3];
+    t_medium_consts(p, c);
+    for (uint32_t i = 0; i < count; ++i) x = pfc::medium_mul(x, n[i], 4u * n[i], c[0], c[1], c[2]);
+    return x;
+}
 // Candidates of [lo, hi) (lo >= 114) exactly as k_generate's lanes compute them.
 size_t t_generate(uint64_t lo, uint64_t hi, uint64_t* out, size_t cap) {
     size_t n = 0;

@@ -661,6 +661,54 @@ def test_primorial_triples_find_planted_factors(orc, dev):
         assert (g.checksum, g.primecount, g.factors) == ref
 
 
+@pytest.fixture(scope="module")
+def planted_narrow(orc):
+    """Jobs around the planted factors of tests/golden/narrow_factors.json
+    (p in (2^34, 2^52), n < 200) with the oracle's results: every mode that
+    walks the factor's chain (F or C, and F|C), from nmin = 101 and 104
+    (which puts the hit at the tested and at the untested step of a pair in
+    both chains), n up to 300."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "narrow_factors.json")
+    with open(path) as f:
+        vecs = json.load(f)["vectors"]
+    descs, planted = [], []
+    for v in vecs:
+        chain = F if v["kind"] == "F" else Cm
+        for modes in (chain, F | Cm):
+            for nmin in (101, 104):
+                if v["n"] >= nmin:
+                    descs.append((modes, v["p"] - 2000, v["p"] + 2000, nmin, 300))
+                    planted.append((v["p"], v["c"] * v["n"], 0 if v["kind"] == "F" else 2))
+    refs = []
+    for desc in descs:
+        r = orc.run(*desc)
+        refs.append((r.checksum, r.primecount, [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in r.factors]))
+    return descs, planted, refs
+
+
+@pytest.mark.parametrize("forms_off,ppl", [(0, 1), (0, 2), (0, 4), (1, 1), (1, 4), (3, 1)])
+def test_walk_forms_find_planted_factors(dev, planted_narrow, forms_off, ppl):
+    """The planted jobs give the oracle's results, with the planted record
+    among them, in each walk form: narrow (plain residues below 2p, the form
+    these moduli take), medium (narrow form off: residues at half the shifted
+    scale) and shifted (both off), with 1, 2 and 4 candidates per lane, run
+    alone and batched."""
+    import pfcsieve_amd as PK
+    descs, planted, refs = planted_narrow
+    dev.set_tuning(primes_per_lane=ppl, walk_forms_off=forms_off)
+    try:
+        for desc, rec, ref in zip(descs, planted, refs):
+            got = PK.run_sieve(*desc, device=dev, verify_threads=8)
+            assert (got.checksum, got.primecount, got.factors) == ref, desc
+            assert rec in got.factors, (desc, rec)
+        for desc, g, ref in zip(descs, PK.run_batch(descs, device=dev, verify_threads=8), refs):
+            assert (g.checksum, g.primecount, g.factors) == ref, desc
+    finally:
+        dev.set_tuning()
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("modes,lo,width,nmin,nmax,cut", [
     (F | Cm, 10**12, 2 * 10**8, 101, 10**6, 73_212_347),            # the -! -c benchmark shape

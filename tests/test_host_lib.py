@@ -207,6 +207,15 @@ def A():
     L.t_walk_attention.argtypes = [u64, u64, u64]
     L.t_mulmod64_nofix.argtypes = [u64, u64, u64, C.POINTER(C.c_int)]
     L.t_mulmod64_nofix.restype = u64
+    L.t_narrow_consts.argtypes = [u64, C.POINTER(u32 * 3)]
+    L.t_narrow_fits.argtypes = [u64, u32]
+    L.t_narrow_walk.argtypes = [u64, u64, C.POINTER(u32), u32]
+    L.t_narrow_walk.restype = u64
+    L.t_narrow_attention.argtypes = [u64, u64, u64]
+    L.t_medium_consts.argtypes = [u64, C.POINTER(u32 * 3)]
+    L.t_medium_fits.argtypes = [u64, u32]
+    L.t_medium_walk.argtypes = [u64, u64, C.POINTER(u32), u32]
+    L.t_medium_walk.restype = u64
     return L
 
 
@@ -324,6 +333,174 @@ def test_walk_step_against_bigint(A):
             assert raw.value in (want << s, (want << s) + d), (p, a, n)
             if raw.value >= d or want in (1, p - 1):
                 assert att.value, (p, a, n)
+
+
+def _narrow_moduli(rng, k):
+    """Odd moduli across the narrow range (2^34, 2^55): both ends, near powers
+    of two, and random bit lengths."""
+    out = [2**34 + 1, 2**34 + 3, 2**35 - 1, 2**40 - 87, 10**12 + 39, 2**48 + 21, 2**55 - 55, 2**55 - 1,
+           (2**41 // 3) | 1, 2**52 - 1]
+    for _ in range(k):
+        b = rng.randrange(35, 56)
+        out.append(rng.randrange(2**(b - 1) + 1, 2**b) | 1)
+    return [p for p in out if 2**34 < p < 2**55]
+
+
+def _narrow_bmax(p):
+    """The largest multiplier the narrow walk takes for p: p * n < 2^61, n < 2^22."""
+    return min((2**61 - 1) // p, 2**22 - 1)
+
+
+def test_narrow_fits_bounds(A):
+    """narrow_fits holds exactly for 2^34 < p < 2^55, 1 <= bmax < 2^22 and
+    p * bmax < 2^61."""
+    rng = random.Random(21)
+    for p in _narrow_moduli(rng, 300) + [2**34 - 1, 2**34, 2**55, 2**55 + 1, 2**60 + 1]:
+        for b in [0, 1, 2, 2**22 - 1, 2**22, 2**32 - 1, (2**61 - 1) // p, (2**61 - 1) // p + 1,
+                  rng.randrange(1, 2**22)]:
+            b = min(b, 2**32 - 1)
+            want = 2**34 < p < 2**55 and 1 <= b < 2**22 and p * b < 2**61
+            assert A.t_narrow_fits(p, b) == int(want), (p, b)
+
+
+def test_narrow_constants(A):
+    """m = floor(2^64 / p) from the 3-by-2 reciprocal, np0 = 2^32 - p0 and
+    kp1 = -(p1 + 1) mod 2^32 (a signed 24-bit value)."""
+    rng = random.Random(22)
+    out = (C.c_uint32 * 3)()
+    for p in _narrow_moduli(rng, 2000):
+        A.t_narrow_consts(p, C.byref(out))
+        assert out[0] == R // p, p
+        assert out[1] == (1 << 32) - (p & 0xFFFFFFFF), p
+        assert out[2] == (-((p >> 32) + 1)) % (1 << 32), p
+        assert (p >> 32) + 1 <= 1 << 23
+
+
+def test_narrow_step_against_bigint(A):
+    """The narrow step maps every X in [0, 2p) and n up to the largest the
+    walk takes to a value in [0, 2p) congruent to X * n mod p; checked at the
+    ends of both ranges (X = 0, 1, p - 1, p, 2p - 1; n = 1, 2, the maximum)
+    and on chains of random multipliers, each step against big integers."""
+    rng = random.Random(23)
+    for p in _narrow_moduli(rng, 1500):
+        bmax = _narrow_bmax(p)
+        xs = [0, 1, 2, p - 1, p, p + 1, 2 * p - 2, 2 * p - 1] + [rng.randrange(2 * p) for _ in range(6)]
+        ns = [1, 2, 3, bmax, bmax - 1, max(1, bmax // 2)] + [rng.randrange(1, bmax + 1) for _ in range(4)]
+        for x in xs:
+            for n in ns:
+                got = A.t_narrow_walk(p, x, (C.c_uint32 * 1)(n), 1)
+                assert 0 <= got < 2 * p and got % p == x * n % p, (p, x, n, got)
+        chain = [rng.randrange(1, bmax + 1) for _ in range(64)]
+        x = rng.randrange(2 * p)
+        got = A.t_narrow_walk(p, x, (C.c_uint32 * 64)(*chain), 64)
+        want = x
+        for n in chain:
+            want = want * n % p
+        assert got < 2 * p and got % p == want, (p, x)
+
+
+def test_narrow_step_near_multiples(A):
+    """Products t = X * n just below and above multiples of p (where a wrong
+    quotient estimate would show), including t close to 2^62."""
+    rng = random.Random(24)
+    for p in _narrow_moduli(rng, 800):
+        bmax = _narrow_bmax(p)
+        for _ in range(12):
+            n = rng.choice([bmax, rng.randrange(1, bmax + 1)])
+            k = rng.randrange(1, 2 * n)                  # a multiple k * p below 2p * n
+            for delta in (-2, -1, 0, 1, 2):
+                t = k * p + delta
+                if t < 0 or t % n:
+                    continue
+                x = t // n
+                if x >= 2 * p:
+                    continue
+                got = A.t_narrow_walk(p, x, (C.c_uint32 * 1)(n), 1)
+                assert got < 2 * p and got % p == t % p, (p, x, n)
+            # X chosen so that X * n mod p lands on 1 and p - 1 (hits), from both halves of [0, 2p)
+            if math.gcd(n, p) == 1:
+                for target in (1, p - 1):
+                    x = target * pow(n, -1, p) % p
+                    for xx in (x, x + p):
+                        got = A.t_narrow_walk(p, xx, (C.c_uint32 * 1)(n), 1)
+                        assert got < 2 * p and got % p == target, (p, xx, n)
+
+
+def test_narrow_attention_bounds(A):
+    """The narrow band test passes exactly the reduced x <= B and x >= p - B,
+    for B = 1, pair bands and triple bands; checked at every interval edge
+    and at random x."""
+    rng = random.Random(25)
+    for p in _narrow_moduli(rng, 400):
+        bmax = _narrow_bmax(p)
+        for B in [1, bmax, rng.randrange(1, bmax + 1), min(bmax * bmax, (p - 2) // 2)]:
+            edges = [0, 1, B - 1, B, B + 1, p - B - 1, p - B, p - B + 1, p - 2, p - 1]
+            for x in edges + [rng.randrange(p) for _ in range(30)]:
+                if not 0 <= x < p:
+                    continue
+                assert A.t_narrow_attention(p, B, x) == int(x <= B or x >= p - B), (p, B, x)
+
+
+def _medium_moduli(rng, k):
+    """Odd moduli across the medium range (2^32, 2^63): both ends, every
+    shift, near powers of two."""
+    out = [2**32 + 1, 2**32 + 15, 2**33 - 9, 10**12 + 39, 2**55 - 55, 2**61 - 1, 2**62 - 57, 2**62 + 135,
+           2**63 - 25, 2**63 - 1, 4611686014427387903]
+    for _ in range(k):
+        b = rng.randrange(33, 64)
+        out.append(rng.randrange(2**(b - 1) + 1, 2**b) | 1)
+    return [p for p in out if 2**32 < p < 2**63]
+
+
+def test_medium_fits_and_constants(A):
+    """medium_fits holds exactly for 2^32 < p < 2^63 and 1 <= bmax < 2^28;
+    M = floor(2^94 / D) for D = p << (s - 1), nd0 = 2^32 - D0, k1 = -(D1 + 1)."""
+    rng = random.Random(31)
+    out = (C.c_uint32 * 3)()
+    for p in _medium_moduli(rng, 1500):
+        for b in [0, 1, 2**28 - 1, 2**28, 2**31]:
+            assert A.t_medium_fits(p, b) == int(1 <= b < 2**28), (p, b)
+        D = p << (64 - p.bit_length() - 1)
+        assert 2**62 < D < 2**63
+        A.t_medium_consts(p, C.byref(out))
+        assert out[0] == 2**94 // D, p
+        assert out[1] == (1 << 32) - (D & 0xFFFFFFFF), p
+        assert out[2] == (-((D >> 32) + 1)) % (1 << 32), p
+    for p in [2**32 - 5, 2**32 + 1 - 2, 2**63 + 29, 2**64 - 59]:
+        assert A.t_medium_fits(p, 1000) == int(2**32 < p < 2**63), p
+
+
+def test_medium_step_against_bigint(A):
+    """The medium step maps every x = X << (s - 1) below 2D and n < 2^28 to a
+    value below 2D congruent to (X * n mod p) << (s - 1): at the ends of both
+    ranges, next to multiples of D, and on chains of random multipliers,
+    each against big integers."""
+    rng = random.Random(32)
+    for p in _medium_moduli(rng, 1500):
+        s1 = 64 - p.bit_length() - 1
+        D = p << s1
+        Xs = [0, 1, 2, p - 1, p, p + 1, 2 * p - 1] + [rng.randrange(2 * p) for _ in range(6)]
+        ns = [1, 2, 3, 2**28 - 1, 2**27 + 1, 65536, 1000003] + [rng.randrange(1, 2**28) for _ in range(3)]
+        for X in Xs:
+            for n in ns:
+                got = A.t_medium_walk(p, X << s1, (C.c_uint32 * 1)(n), 1)
+                assert got < 2 * D and got % (1 << s1) == 0 and (got >> s1) % p == X * n % p, (p, X, n)
+        for _ in range(6):                          # X * n next to a multiple of p
+            n = rng.choice([2**28 - 1, rng.randrange(1, 2**28)])
+            k = rng.randrange(1, 2 * n)
+            for delta in (-1, 0, 1):
+                t = k * p + delta
+                if t > 0 and t % n == 0 and t // n < 2 * p:
+                    X = t // n
+                    got = A.t_medium_walk(p, X << s1, (C.c_uint32 * 1)(n), 1)
+                    assert got < 2 * D and (got >> s1) % p == t % p, (p, X, n)
+        chain = [rng.randrange(1, 2**28) for _ in range(64)]
+        X = rng.randrange(2 * p)
+        got = A.t_medium_walk(p, X << s1, (C.c_uint32 * 64)(*chain), 64)
+        want = X
+        for n in chain:
+            want = want * n % p
+        assert got < 2 * D and (got >> s1) % p == want, (p, X)
 
 
 def test_strong_prp2_device_arith(A, orc):

@@ -320,3 +320,33 @@ def test_primorial_factor_fixtures(orc):
         r = orc.run(2, p - 200, p + 200, 101, 400)
         got = [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in r.factors]
         assert (p, c * n, 1) in got, v
+
+
+def test_narrow_factor_fixtures(orc):
+    """The planted factors of the narrow walk (tests/golden/narrow_factors.json):
+    each p is prime, lies in (2^34, 2^52), divides n! + c (kind F) or
+    n!/n# + c (kind C, n composite) by big integers, and the oracle reports
+    it in a small p-window around it; the fixtures have hits of both signs
+    at odd and even n in both chains."""
+    import json
+    import math
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "narrow_factors.json")
+    with open(path) as f:
+        d = json.load(f)
+    assert (d["nmin"], d["nmax"]) == (101, 200)
+    seen = set()
+    for v in d["vectors"]:
+        kind, n, c, p = v["kind"], v["n"], v["c"], v["p"]
+        assert sympy.isprime(p) and 2**34 < p < 2**52
+        x = math.factorial(n)
+        if kind == "C":
+            assert not sympy.isprime(n)
+            x //= sympy.primorial(n, nth=False)
+        assert (x + c) % p == 0, v
+        r = orc.run(1 if kind == "F" else 4, p - 200, p + 200, 101, 300)
+        got = [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in r.factors]
+        assert (p, c * n, 0 if kind == "F" else 2) in got, v
+        seen.add((kind, n % 2, c))
+    assert {(k, par) for k, par, _ in seen} == {("F", 0), ("F", 1), ("C", 0), ("C", 1)}
+    assert {c for _, _, c in seen} == {-1, 1}
