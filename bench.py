@@ -398,6 +398,47 @@ def kernel_rates(dev, reps=3):
     return out
 
 
+def walk_forms(dev):
+    """The walk of one p-segment (p-width 1e8) in each arithmetic form its
+    moduli allow (pfc_tuning.walk_forms_off): -! -c at n in [101, 2e5) and
+    [9e5, 1e6) from p = 1e12 (narrow, medium, shifted) and from p = 2^61
+    (medium, shifted), -! at n in [9e5, 1e6) from 2^61, and -# at
+    n in [101, 1e6) from p = 2^62 - 1e8 (medium, shifted); walk launches
+    timed by HIP events, the same candidates and multipliers in every form."""
+    import pfcsieve_amd as P
+    from pfcsieve_amd import workload as Wl
+    out = {}
+    for name, modes, pmin, nmin, nmax, forms in (
+            ("fc_p1e12", 5, 10**12, 101, 200_000, (("narrow", 0), ("medium", 1), ("shifted", 3))),
+            ("fc_p1e12_n9e5", 5, 10**12, 900_000, 1_000_000, (("narrow", 0), ("medium", 1), ("shifted", 3))),
+            ("fc_p2e61", 5, 2**61, 101, 200_000, (("medium", 0), ("shifted", 2))),
+            ("fc_p2e61_n9e5", 5, 2**61, 900_000, 1_000_000, (("medium", 0), ("shifted", 2))),
+            ("f_p2e61_n9e5", 1, 2**61, 900_000, 1_000_000, (("medium", 0), ("shifted", 2))),
+            ("p_p2e62", 2, 2**62 - 10**8, 101, 1_000_000, (("medium", 0), ("shifted", 2)))):
+        rows = {}
+        for form, off in forms:
+            dev.set_tuning(walk_forms_off=off, seg_candidates=1 << 24)
+            try:
+                j = P.Job(dev, modes, pmin, pmin + 10**8, nmin, nmax)
+                j.run()
+                s = j.stats()
+                res = j.result()
+                j.destroy()
+            finally:
+                dev.set_tuning()
+            rows[form] = {"walk_launches": s.walk_launches, "walk_ms": s.walk_ms,
+                          "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3),
+                          "checksum": f"{res[0]:016X}"}
+        assert len({r["checksum"] for r in rows.values()}) == 1, rows
+        out[name] = {"workload": f"modes {modes}, n in [{nmin}, {nmax}), p in [{pmin}, {pmin + 10**8})",
+                     "candidates": s.primecount,
+                     "steps_per_candidate": (Wl.walk_steps(Wl.Unit(1, 0, 0, nmin, nmax))
+                                             + Wl.walk_steps(Wl.Unit(4, 0, 0, nmin, nmax))) if modes == 5
+                     else Wl.walk_steps(Wl.Unit(modes, 0, 0, nmin, nmax)),
+                     "forms": rows}
+    return out
+
+
 def walk_modes(dev):
     """One 2^24-candidate p-segment of each walk mode at the shapes the mixes
     use: -! and -c alone at n in [101, 1e6) from p = 1e12, and -# at
@@ -477,6 +518,7 @@ def main():
     ap.add_argument("--no-headline", action="store_true")
     ap.add_argument("--kernel-rates", action="store_true", help="also time k_generate and k_setup standalone")
     ap.add_argument("--modes", action="store_true", help="also time one p-segment of each walk mode")
+    ap.add_argument("--forms", action="store_true", help="also time one p-segment in each walk arithmetic form")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run one plain device copy of this many bytes first (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -550,6 +592,8 @@ def main():
         line["kernel_rates"] = kernel_rates(dev)
     if args.modes and "walk_modes" not in line:
         line["walk_modes"] = walk_modes(dev)
+    if args.forms:
+        line["walk_forms"] = walk_forms(dev)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_width)
         if head:
