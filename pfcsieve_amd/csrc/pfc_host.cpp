@@ -1642,8 +1642,12 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
         std::stable_sort(setups.begin(), setups.end(),
                          [](const Tmp* a, const Tmp* c) { return a->setup_cost > c->setup_cost; });
         for (const Tmp* t : setups) append(*t, 4);
-        for (const Tmp& t : ranges)
-            if (t.r.flags & 2u) append(t, 5);
+        for (const Tmp& t : ranges) {
+            if (!(t.r.flags & 2u)) continue;
+            Tmp f = t;                     // fold items are kFoldBlocks walk items wide
+            f.count = (t.count + pfc::kFoldBlocks - 1) / pfc::kFoldBlocks;
+            append(f, 5);
+        }
         rc = ensure_device(&b->d_jobs, &b->d_jobs_cap, ns);
         if (!rc) rc = ensure_device(&b->d_ranges, &b->d_ranges_cap, flat.size());
         if (rc) return rc;

@@ -85,6 +85,9 @@ int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint3
 // launch_gather_ctrl).  d_ranges holds the phases' range lists in phase
 // order, each with its own item ends.
 constexpr uint32_t kBatchWords = 16;
+// A fold item covers kFoldBlocks blocks of 64 * K candidates (a walk or
+// set-up item, one block).
+constexpr uint32_t kFoldBlocks = 16;
 int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, uint32_t* d_words,
                  uint32_t ppl, const uint32_t nblocks[6], hipStream_t st);
 // Clears the six queue heads of d_words.

@@ -1281,30 +1281,35 @@ __device__ __forceinline__ void batch_walk_item(const pfc_batch_job& J, const pf
     }
 }
 
-// Checksum terms of an item's candidates (as k_check, with the same check of
-// every stored reciprocal); the item at candidate 0 also counts the segment's
-// candidates.
+// Checksum terms of a fold item's candidates, kFoldBlocks blocks of 64 * K
+// (as k_check, with the same check of every stored reciprocal), added to the
+// job's control block with one atomic per item; the item at candidate 0 also
+// counts the segment's candidates.  A fold item is kFoldBlocks times a walk
+// item: one atomic per 64 candidates on each job's checksum had set the
+// light fold's pace.
 template <int K>
 __device__ __forceinline__ void batch_fold_item(const pfc_batch_job& J, const pfc_batch_item& it) {
     uint32_t n_c = J.ctrl->ncand;
     if (n_c > J.cap) n_c = J.cap;
-    const uint32_t first = it.first + __lane_id() * K;
     const bool two = J.modes == (PFC_MODE_FACTORIAL | PFC_MODE_COMPOSITORIAL);
     uint64_t sum = 0;
     bool bad = false;
+    for (uint32_t blk = 0; blk < pfc::kFoldBlocks; ++blk) {
+        const uint32_t first = it.first + blk * (64u * K) + __lane_id() * K;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const uint32_t idx = first + k;
-        if (idx >= n_c) continue;
-        const uint64_t p = J.P[idx];
-        Modulus m;
-        m.s = pfc::clz64(p);
-        const uint64_t d = p << m.s;
-        m.d1 = (uint32_t)(d >> 32);
-        m.d0 = (uint32_t)d;
-        m.v = J.V[idx];
-        bad |= !pfc::modulus_ok(p, m);
-        sum += check_term(J.modes, J.nmax, m, J.ONE[idx], J.RA[idx], two ? J.RB[idx] : 0);
+        for (int k = 0; k < K; ++k) {
+            const uint32_t idx = first + k;
+            if (idx >= n_c) continue;
+            const uint64_t p = J.P[idx];
+            Modulus m;
+            m.s = pfc::clz64(p);
+            const uint64_t d = p << m.s;
+            m.d1 = (uint32_t)(d >> 32);
+            m.d0 = (uint32_t)d;
+            m.v = J.V[idx];
+            bad |= !pfc::modulus_ok(p, m);
+            sum += check_term(J.modes, J.nmax, m, J.ONE[idx], J.RA[idx], two ? J.RB[idx] : 0);
+        }
     }
     if (__ballot(bad) != 0 && __lane_id() == 0) atomicOr(&J.ctrl->flags, 2u);
     wave_add_u64(sum, &J.ctrl->checksum);
@@ -1330,6 +1335,11 @@ __global__ __launch_bounds__(64) void k_batch(const pfc_batch_job* __restrict__ 
     const pfc_batch_range* ranges = all_ranges + base;
     uint32_t* head = words + PHASE;
     const uint32_t nitems = ranges[nranges - 1].end;
+    // Every phase takes its items from its queue in order, one per update, so
+    // that the wavefronts of a CU work on neighbouring items (the set-up's
+    // start tables and the walks' multiplier lists are shared through the
+    // caches).
+    const uint32_t item_cands = 64u * K * (PHASE == kPhaseFold ? pfc::kFoldBlocks : 1u);
     for (;;) {
         uint32_t i = 0;
         if (__lane_id() == 0) i = atomicAdd(head, 1u);
@@ -1345,7 +1355,7 @@ __global__ __launch_bounds__(64) void k_batch(const pfc_batch_job* __restrict__ 
         const uint32_t start = lo ? ranges[lo - 1].end : 0u;
         pfc_batch_item it;
         it.job = R.job;
-        it.first = (i - start) * (64u * K);
+        it.first = (i - start) * item_cands;
         it.j0 = R.j0;
         it.j1 = R.j1;
         it.flags = R.flags;
