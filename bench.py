@@ -12,8 +12,9 @@ Workloads (synthetic: every candidate prime is generated on the device):
   from 101+U[0,1e6], p-width log-U[1e6,1e9] from log-U[1e10,2^63]), every
   unit one p-segment.  Batched: all units in one batcher, timed from job
   creation to job destruction; a step is one batcher round (its device
-  phase -- work-item upload, queue clear, one persistent k_batch launch,
-  control-block readback -- replayed from one captured HIP graph), and the K
+  phase -- work-item upload, queue clear, the persistent set-up, per-mode
+  walk and fold launches, control-block readback -- replayed from one
+  captured HIP graph), and the K
   rounds are sized by work (round r walks about
   the remaining updates / (K - r); the last walks everything left), so every
   step does work and the K steps cover the mix.  Plain (``mix.plain``): the
@@ -314,28 +315,46 @@ def stream_bound(units, rates):
     return t
 
 
-def serve_prefix(dev, njobs, rates, round_updates, progress):
-    """The first njobs of the seeded 10,000-job stream through the serving
-    loop.  Poisson arrivals at 0.8 x the capacity the stream's own device-work
+def serve_prefix(dev, njobs, rates, round_updates, progress, start=0, full_rate=False, latencies_path=None,
+                 rate=0.0):
+    """Jobs [start, start + njobs) of the seeded 10,000-job stream through the
+    serving loop.  Poisson arrivals at 0.8 x the capacity that a device-work
     bound gives (stream_bound at the measured kernel rates), instead of a
-    separate capacity run."""
+    separate capacity run: the bound of the jobs served, or with full_rate
+    (and for any slice not starting at job 0) that of the whole stream, so
+    that slices of it run at the whole stream's rate.  A slice keeps the
+    stream's own arrival times, shifted so that its first job arrives one
+    inter-arrival gap after the clock starts; it starts with nothing
+    resident.  latencies_path: write every job's (index, arrival, latency)
+    as JSON there.  rate > 0 fixes the arrival rate (jobs/s) instead."""
     import pfcsieve_amd.serve as S
     from pfcsieve_amd import workload as Wl
-    units = Wl.mix(10_000, seed=2024)[:njobs]
+    stream = Wl.mix(10_000, seed=2024)
+    units = stream[start:start + njobs]
+    full = full_rate or start > 0
+    bound_all = stream_bound(stream, rates) if full else None
     bound = stream_bound(units, rates)
-    lam = 0.8 * njobs / bound
-    arrivals = Wl.poisson_arrivals(njobs, lam, seed=777)
+    lam = rate if rate > 0 else 0.8 * len(stream) / bound_all if full else 0.8 * njobs / bound
+    arrivals = Wl.poisson_arrivals(start + njobs, lam, seed=777)
+    shift = arrivals[start - 1] if start > 0 else 0.0
+    arrivals = [a - shift for a in arrivals[start:start + njobs]]
     info = dev.info()
     say = (lambda m: _progress(True, m)) if progress else None
     srv = S.Server(dev, hbm_budget=int(info["hbm_bytes"] * 0.8), verify_threads=16, progress=say,
                    round_updates=round_updates)
     for k, (u, a) in enumerate(zip(units, arrivals)):
-        srv.submit(S.Request(k, u.desc(), a))
+        srv.submit(S.Request(start + k, u.desc(), a))
     st = srv.run()
-    upd = sum(r.result.primecount * Wl.walk_steps(units[r.rid]) for r in srv.done)
-    return {"jobs": st.jobs, "stream": f"first {njobs} of 10,000 seeded jobs (splitmix64 seed 2024, units as in "
-                                       f"the mix), Poisson arrivals (seed 777)",
-            "device_work_bound_s": bound, "arrival_rate_per_s": lam, "last_arrival_s": arrivals[-1],
+    upd = sum(r.result.primecount * Wl.walk_steps(stream[r.rid]) for r in srv.done)
+    if latencies_path:
+        with open(latencies_path, "w") as f:
+            json.dump([[r.rid, r.arrival, r.finished - r.arrival] for r in srv.done], f)
+    which = f"first {njobs}" if start == 0 else f"jobs [{start}, {start + njobs})"
+    return {"jobs": st.jobs, "stream": f"{which} of 10,000 seeded jobs (splitmix64 seed 2024, units as in "
+                                       f"the mix), Poisson arrivals (seed 777)"
+                                       + (" at the whole stream's rate" if full else ""),
+            "device_work_bound_s": bound, "device_work_bound_stream_s": bound_all,
+            "arrival_rate_per_s": lam, "last_arrival_s": arrivals[-1],
             "wall_s": st.wall_s, "rounds": st.rounds, "updates_per_s": upd / st.wall_s,
             "jobs_per_s": st.jobs / st.wall_s, "p50_latency_s": st.pct(50), "p99_latency_s": st.pct(99),
             "max_latency_s": max(st.latencies) if st.latencies else None,
@@ -514,6 +533,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mix-units", type=int, default=256, help="units of the seeded mix (256 = the whole mix)")
     ap.add_argument("--serve-jobs", type=int, default=200, help="jobs of the seeded served stream (0 = skip)")
+    ap.add_argument("--serve-from", type=int, default=0, help="first job of the served slice of the stream")
+    ap.add_argument("--serve-full-rate", action="store_true",
+                    help="arrival rate from the whole 10,000-job stream's device-work bound (slices)")
+    ap.add_argument("--serve-latencies", default="", help="write every served job's latency to this JSON file")
+    ap.add_argument("--serve-rate", type=float, default=0.0, help="fixed arrival rate, jobs/s (0: from the bound)")
     ap.add_argument("--serve-round-updates", type=float, default=4e11)
     ap.add_argument("--no-headline", action="store_true")
     ap.add_argument("--kernel-rates", action="store_true", help="also time k_generate and k_setup standalone")
@@ -548,9 +572,10 @@ def main():
         "dtype": "u64 (exact modular integer)", "data": "synthetic (seeded job mix; device-generated primes)",
         "config": {"workload": (f"batched uneven mix: {mix['units']} seeded units (splitmix64 seed 12345)"
                                 if mix else "headline -! -c job" if head else "kernel rates only"),
-                   "form": "batched: all units through one batcher, one persistent k_batch launch per round "
-                           "replayed from a captured HIP graph; a step is one batcher round, the K rounds sized "
-                           "by work; plain (one job after another, eager) beside it under 'mix'",
+                   "form": "batched: all units through one batcher, each round's persistent set-up, per-mode "
+                           "walk and fold launches replayed from a captured HIP graph; a step is one batcher "
+                           "round, the K rounds sized by work; plain (one job after another, eager) beside it "
+                           "under 'mix'",
                    "mix": mix, "headline": head, "tuning": dev.tuning()},
     }
     if head:
@@ -585,7 +610,9 @@ def main():
         rates = {"walk": kr["walk_medium_p2e61"]["walk_updates_per_s"], "walk_narrow": walk_narrow,
                  "setup": kr["setup_nmin_1e6"]["products_per_s"],
                  "generate": kr["generate_p1e12_2e9"]["candidates_per_s"]}
-        line["served"] = serve_prefix(dev, args.serve_jobs, rates, int(args.serve_round_updates), args.progress)
+        line["served"] = serve_prefix(dev, args.serve_jobs, rates, int(args.serve_round_updates), args.progress,
+                                      start=args.serve_from, full_rate=args.serve_full_rate,
+                                      latencies_path=args.serve_latencies or None, rate=args.serve_rate)
         line["served"]["rates"] = rates
     line["config"]["small_job"] = small_job(dev)
     if args.kernel_rates and "kernel_rates" not in line:

@@ -76,8 +76,8 @@ typedef struct pfc_tuning {
     uint32_t use_graph;        /* 1: capture a job's segment pipeline (clear, generate, setup,
                                   walk chunks, check) once as a HIP graph and replay it per segment;
                                   and a batcher round's device phase (work-item upload, queue clear,
-                                  the persistent set-up + walk + fold launch, control-block readback)
-                                  once per batcher, replayed every round */
+                                  the persistent set-up, walk and fold launches, control-block
+                                  readback) once per batcher, replayed every round */
     uint32_t skip_table_check; /* 1: do not self-check the start tables at job creation
                                   (pfc_stage_verify_table); 0 (default): check them */
     uint32_t lds_segment;      /* integers sieved per generator workgroup and staged in LDS:
@@ -168,8 +168,9 @@ int pfc_job_debug_shorten_walk(pfc_job* job, uint32_t steps);
  * sequence.  A batcher runs many jobs together: each round generates new
  * p-segments for jobs that need one, cuts every active segment into work
  * items (64*K candidates x up to max_steps multiplier steps, longest first)
- * and drains them with ONE launch of a persistent kernel fed by a device work
- * queue (set-up, walk and checksum fold fused).  Job state stays resident on
+ * and drains them with persistent kernels fed by device work queues: one
+ * launch for the set-up of new segments, one per walk mode for the walk, one
+ * for the checksum fold of finished segments.  Job state stays resident on
  * the device between rounds; jobs can be added between rounds (continuous
  * batching).  Results land in the jobs exactly as pfc_job_run leaves them. */
 typedef struct pfc_batcher pfc_batcher;
@@ -301,8 +302,11 @@ int pfc_stage_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, 
  * and this call leave them.  (Where every p of a wavefront exceeds twice the
  * largest multiplier j1 - 1, F, C and F|C steps are tested in pairs, and a
  * pair with both multipliers at most 65,536 is one step by their product;
- * P steps are tested one by one here, since the call cannot bound d_list's
- * values.  The records are the same either way.) */
+ * such a wavefront walks plain residues below 2p when 2^34 < p < 2^55 and
+ * p * (j1 - 1) < 2^61, or half-scale residues when p < 2^63, converting on
+ * entry and returning them canonical.  P steps are tested one by one here,
+ * in the shifted form, since the call cannot bound d_list's values.  The
+ * records are the same either way.) */
 int pfc_stage_walk(uint32_t modes, uint32_t j0, uint32_t j1, const uint32_t* d_list,
                    const uint32_t* d_composite, uint32_t nbase, const uint64_t* d_p,
                    const uint32_t* d_v, uint64_t* d_res_a, uint64_t* d_res_b, pfc_ctrl* d_ctrl,

@@ -1249,11 +1249,12 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
 //   2. the candidates of every active segment are cut into work items of
 //      64*K candidates x up to `max_steps` multiplier steps, described by one
 //      range per segment and ordered by decreasing cost (longest first), and
-//      ONE launch of the persistent k_batch kernel drains them from a device
-//      queue (a counter; each item finds its segment by binary search);
-//   3. segments whose walk finished (their last items carried the checksum
-//      fold) are drained to the host like pfc_job_run does; finished jobs are
-//      returned to the caller.
+//      persistent kernels drain them from device queues (a counter each;
+//      an item finds its segment by binary search): the set-up of segments
+//      that start in this round, the walk of each walk mode, the checksum
+//      fold of segments whose walk ends in this round;
+//   3. finished segments are drained to the host like pfc_job_run does;
+//      finished jobs are returned to the caller.
 // Jobs can be added between rounds; their state stays resident on the device.
 
 // Work items per batch launch at most (the queue counter is 32-bit); segments
