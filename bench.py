@@ -358,6 +358,8 @@ def serve_prefix(dev, njobs, rates, round_updates, progress, start=0, full_rate=
             "wall_s": st.wall_s, "rounds": st.rounds, "updates_per_s": upd / st.wall_s,
             "jobs_per_s": st.jobs / st.wall_s, "p50_latency_s": st.pct(50), "p99_latency_s": st.pct(99),
             "max_latency_s": max(st.latencies) if st.latencies else None,
+            "host_split_s": {"admit": st.admit_s, "rounds": st.round_s, "harvest": st.harvest_s,
+                             "idle": st.idle_s},
             "peak_resident_bytes": st.peak_resident_bytes, "round_policy": f"about {round_updates:.3g} p*n "
                                                                            f"updates per round"}
 

@@ -86,6 +86,9 @@ typedef struct pfc_tuning {
                                   narrow form (plain residues, 2^34 < p < 2^55, p * n < 2^61), bit 1
                                   the medium form (2^32 < p < 2^63, n < 2^28); 0 (default): each
                                   launch takes the cheapest form its primes and multipliers allow */
+    uint32_t batch_walk_blocks; /* blocks of 64 * primes_per_lane candidates per batcher walk item,
+                                  1 to 64; 0 (default): as many as bring an item's steps x blocks
+                                  to 32768 while the round keeps four items per wavefront slot */
 } pfc_tuning;
 
 /* Host-side summary of a finished job (cl_sieve.cpp:1748). */

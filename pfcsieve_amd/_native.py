@@ -41,7 +41,7 @@ class Tuning(C.Structure):
     _fields_ = [("primes_per_lane", C.c_uint32), ("block_size", C.c_uint32),
                 ("walk_chunk", C.c_uint32), ("seg_candidates", C.c_uint32), ("use_graph", C.c_uint32),
                 ("skip_table_check", C.c_uint32), ("lds_segment", C.c_uint32),
-                ("walk_forms_off", C.c_uint32)]
+                ("walk_forms_off", C.c_uint32), ("batch_walk_blocks", C.c_uint32)]
 
 
 class JobStats(C.Structure):

@@ -351,6 +351,7 @@ static pfc_tuning default_tuning() {
     t.skip_table_check = 0;
     t.lds_segment = 15360;   // 256 threads x one 60-integer window
     t.walk_forms_off = 0;
+    t.batch_walk_blocks = 0;
     return t;
 }
 
@@ -410,6 +411,8 @@ extern "C" int pfc_set_tuning(pfc_device* dev, const pfc_tuning* t) {
     if (n.lds_segment != 15360 && n.lds_segment != 30720 && n.lds_segment != 61440)
         return set_error(PFC_ERR_INVALID, "lds_segment must be 15360, 30720 or 61440");
     if (n.walk_forms_off > 3) return set_error(PFC_ERR_INVALID, "walk_forms_off must be 0 to 3");
+    if (n.batch_walk_blocks > pfc::kWalkMaxBlocks)
+        return set_error(PFC_ERR_INVALID, "batch_walk_blocks must be 0 to 64");
     std::lock_guard<std::mutex> g(dev->mu);
     dev->tuning = n;
     return PFC_OK;
@@ -1583,15 +1586,24 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
     // ranges ordered by their items' cost, longest first, so that the queue
     // ends on short items.
     std::vector<pfc_batch_job> jobs(ns);
-    struct Tmp { pfc_batch_range r; uint32_t count; double cost, setup_cost; };
+    struct Tmp { pfc_batch_range r; uint32_t count; double cost, setup_cost; uint32_t ncand; };
     std::vector<Tmp> ranges;
-    uint32_t K;
+    uint32_t K, forced_blocks;
     {
         std::lock_guard<std::mutex> g(b->dev->mu);
         K = b->dev->tuning.primes_per_lane;
+        forced_blocks = b->dev->tuning.batch_walk_blocks;
     }
     std::vector<uint8_t> finishing(ns, 0);
     uint64_t nitems = 0;
+    // Walk items grow to several blocks only while the round keeps at least
+    // four items per launched wavefront (the queue's tail stays short).
+    uint64_t walk_blocks = 0;
+    for (const pfc_batcher::Slot& s : b->slots)
+        if (s.stage == 1) walk_blocks += (s.ncand + 64ull * K - 1) / (64ull * K);
+    const uint64_t waves = (uint64_t)b->dev->prop.multiProcessorCount * 32;
+    const uint32_t blocks_cap =
+        (uint32_t)std::min<uint64_t>(pfc::kWalkMaxBlocks, std::max<uint64_t>(1, walk_blocks / (4 * waves)));
     for (size_t i = 0; i < ns; ++i) {
         pfc_batcher::Slot& s = b->slots[i];
         jobs[i] = batch_view(s.job);
@@ -1606,14 +1618,21 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
             s.jcur = j->jend;
             continue;
         }
-        const uint32_t count = (uint32_t)((s.ncand + 64ull * K - 1) / (64ull * K));
+        // A walk item covers enough blocks of 64*K candidates that steps x
+        // blocks reaches kWalkItemSteps: short rounds keep their items big.
+        const uint32_t steps = std::max<uint32_t>(1u, jw - j0);
+        const uint32_t blocks =
+            forced_blocks ? forced_blocks : std::min<uint32_t>(blocks_cap, (pfc::kWalkItemSteps + steps - 1) / steps);
+        const uint64_t per_item = 64ull * K * blocks;
+        const uint32_t count = (uint32_t)((s.ncand + per_item - 1) / per_item);
         if (nitems + count > kMaxBatchItems) continue;   // left for the next round
         nitems += count;
         // Relative cost of one walk item: walk steps (the F|C walk takes ~1.9
-        // steps per n).  The set-up list orders by start-table size.
-        const double per = (double)(j1 - j0) * ((j->d_rb) ? 1.9 : 1.0);
-        ranges.push_back({{(uint32_t)i, j0, jw, flags, walk_bound(j, jw), 0, {0, 0}}, count, per,
-                          (double)(j->ta.chunks.size() + j->tb.chunks.size())});
+        // steps per n) times its blocks.  The set-up list orders by
+        // start-table size.
+        const double per = (double)(j1 - j0) * ((j->d_rb) ? 1.9 : 1.0) * blocks;
+        ranges.push_back({{(uint32_t)i, j0, jw, flags, walk_bound(j, jw), 0, blocks, 0}, count, per,
+                          (double)(j->ta.chunks.size() + j->tb.chunks.size()), s.ncand});
         s.jcur = j1;
         if (j1 == j->jend) finishing[i] = 1;
     }
@@ -1642,13 +1661,17 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
             if (t.r.flags & 1u) setups.push_back(&t);
         std::stable_sort(setups.begin(), setups.end(),
                          [](const Tmp* a, const Tmp* c) { return a->setup_cost > c->setup_cost; });
-        for (const Tmp* t : setups) append(*t, 4);
-        for (const Tmp& t : ranges) {
-            if (!(t.r.flags & 2u)) continue;
-            Tmp f = t;                     // fold items are kFoldBlocks walk items wide
-            f.count = (t.count + pfc::kFoldBlocks - 1) / pfc::kFoldBlocks;
-            append(f, 5);
-        }
+        // set-up items are one block of 64*K candidates, fold items kFoldBlocks
+        auto resized = [&](const Tmp& t, uint32_t blocks) {
+            Tmp f = t;
+            f.r.blocks = blocks;
+            const uint64_t per_item = 64ull * K * blocks;
+            f.count = (uint32_t)((t.ncand + per_item - 1) / per_item);
+            return f;
+        };
+        for (const Tmp* t : setups) append(resized(*t, 1), 4);
+        for (const Tmp& t : ranges)
+            if (t.r.flags & 2u) append(resized(t, pfc::kFoldBlocks), 5);
         rc = ensure_device(&b->d_jobs, &b->d_jobs_cap, ns);
         if (!rc) rc = ensure_device(&b->d_ranges, &b->d_ranges_cap, flat.size());
         if (rc) return rc;

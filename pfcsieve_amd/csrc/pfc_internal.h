@@ -33,7 +33,7 @@ struct pfc_batch_job {
 // multiplier steps [j0, j1) whose largest multiplier is bmax (0: not known);
 // flags bit0 = set up first, bit1 = fold checksum.
 struct pfc_batch_item {
-    uint32_t job, first, j0, j1, flags, bmax, pad[2];
+    uint32_t job, first, j0, j1, flags, bmax, blocks, pad;   // blocks: of 64*K candidates, from first
 };
 
 // The work items of one job segment in one launch: items [start, end) of the
@@ -41,7 +41,9 @@ struct pfc_batch_item {
 // item start + k covers candidates from k * 64*K.  Ranges are stored in queue
 // order with increasing ends.
 struct pfc_batch_range {
-    uint32_t job, j0, j1, flags, bmax, end, pad[2];
+    uint32_t job, j0, j1, flags, bmax, end;
+    uint32_t blocks;           // an item of this range covers blocks x 64*K candidates
+    uint32_t pad;
 };
 
 namespace pfc {
@@ -85,9 +87,12 @@ int launch_check(uint32_t modes, uint32_t nmax, const uint64_t* d_p, const uint3
 // launch_gather_ctrl).  d_ranges holds the phases' range lists in phase
 // order, each with its own item ends.
 constexpr uint32_t kBatchWords = 16;
-// A fold item covers kFoldBlocks blocks of 64 * K candidates (a walk or
-// set-up item, one block).
+// A fold item covers kFoldBlocks blocks of 64 * K candidates, a set-up item
+// one; a walk item as many as make its steps x blocks reach kWalkItemSteps
+// (at most kWalkMaxBlocks), so that short rounds (a serving loop's) do not
+// spend their time on queue updates.
 constexpr uint32_t kFoldBlocks = 16;
+constexpr uint32_t kWalkItemSteps = 32768, kWalkMaxBlocks = 64;
 int launch_batch(const pfc_batch_job* d_jobs, const pfc_batch_range* d_ranges, uint32_t* d_words,
                  uint32_t ppl, const uint32_t nblocks[6], hipStream_t st);
 // Clears the six queue heads of d_words.

@@ -1252,12 +1252,12 @@ __device__ __forceinline__ void batch_setup_item(const pfc_batch_job& J, const p
     }
 }
 
-// The walk of an item's candidates over its multiplier steps (walk mode MODE).
+// The walk of one block of 64 * K candidates from first0 (< the candidate
+// count) over an item's multiplier steps (walk mode MODE).
 template <int K, int MODE>
-__device__ __forceinline__ void batch_walk_item(const pfc_batch_job& J, const pfc_batch_item& it) {
-    uint32_t n_c = J.ctrl->ncand;
-    if (n_c > J.cap) n_c = J.cap;
-    const uint32_t first = it.first + __lane_id() * K;
+__device__ __forceinline__ void batch_walk_block(const pfc_batch_job& J, const pfc_batch_item& it, uint32_t first0,
+                                                 uint32_t n_c) {
+    const uint32_t first = first0 + __lane_id() * K;
     Lanes<K> L;
     uint64_t ra[K], rb[K];
     const bool two = J.modes == (PFC_MODE_FACTORIAL | PFC_MODE_COMPOSITORIAL);
@@ -1265,7 +1265,7 @@ __device__ __forceinline__ void batch_walk_item(const pfc_batch_job& J, const pf
     for (int k = 0; k < K; ++k) {
         const uint32_t idx = first + k;
         L.live[k] = idx < n_c;
-        const uint32_t src = L.live[k] ? idx : it.first;   // items never start past ncand
+        const uint32_t src = L.live[k] ? idx : first0;
         lanes_from_state<K>(L, J.P[src], J.V[src], k);
         ra[k] = J.RA[src];
         rb[k] = two ? J.RB[src] : 0;
@@ -1281,12 +1281,22 @@ __device__ __forceinline__ void batch_walk_item(const pfc_batch_job& J, const pf
     }
 }
 
-// Checksum terms of a fold item's candidates, kFoldBlocks blocks of 64 * K
-// (as k_check, with the same check of every stored reciprocal), added to the
-// job's control block with one atomic per item; the item at candidate 0 also
-// counts the segment's candidates.  A fold item is kFoldBlocks times a walk
-// item: one atomic per 64 candidates on each job's checksum had set the
-// light fold's pace.
+// The walk of an item: its blocks one after another (a uniform loop).
+template <int K, int MODE>
+__device__ __forceinline__ void batch_walk_item(const pfc_batch_job& J, const pfc_batch_item& it) {
+    uint32_t n_c = J.ctrl->ncand;
+    if (n_c > J.cap) n_c = J.cap;
+    for (uint32_t blk = 0; blk < it.blocks; ++blk) {
+        const uint32_t first0 = it.first + blk * (64u * K);
+        if (first0 < n_c) batch_walk_block<K, MODE>(J, it, first0, n_c);
+    }
+}
+
+// Checksum terms of a fold item's candidates, its blocks of 64 * K
+// (pfc::kFoldBlocks; as k_check, with the same check of every stored
+// reciprocal), added to the job's control block with one atomic per item;
+// the item at candidate 0 also counts the segment's candidates.  One atomic
+// per 64 candidates on each job's checksum had set the light fold's pace.
 template <int K>
 __device__ __forceinline__ void batch_fold_item(const pfc_batch_job& J, const pfc_batch_item& it) {
     uint32_t n_c = J.ctrl->ncand;
@@ -1294,7 +1304,7 @@ __device__ __forceinline__ void batch_fold_item(const pfc_batch_job& J, const pf
     const bool two = J.modes == (PFC_MODE_FACTORIAL | PFC_MODE_COMPOSITORIAL);
     uint64_t sum = 0;
     bool bad = false;
-    for (uint32_t blk = 0; blk < pfc::kFoldBlocks; ++blk) {
+    for (uint32_t blk = 0; blk < it.blocks; ++blk) {
         const uint32_t first = it.first + blk * (64u * K) + __lane_id() * K;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -1339,7 +1349,6 @@ __global__ __launch_bounds__(64) void k_batch(const pfc_batch_job* __restrict__ 
     // that the wavefronts of a CU work on neighbouring items (the set-up's
     // start tables and the walks' multiplier lists are shared through the
     // caches).
-    const uint32_t item_cands = 64u * K * (PHASE == kPhaseFold ? pfc::kFoldBlocks : 1u);
     for (;;) {
         uint32_t i = 0;
         if (__lane_id() == 0) i = atomicAdd(head, 1u);
@@ -1355,7 +1364,8 @@ __global__ __launch_bounds__(64) void k_batch(const pfc_batch_job* __restrict__ 
         const uint32_t start = lo ? ranges[lo - 1].end : 0u;
         pfc_batch_item it;
         it.job = R.job;
-        it.first = (i - start) * item_cands;
+        it.first = (i - start) * (64u * K * R.blocks);
+        it.blocks = R.blocks;
         it.j0 = R.j0;
         it.j1 = R.j1;
         it.flags = R.flags;

@@ -40,6 +40,10 @@ class ServeStats:
     rounds: int = 0
     latencies: List[float] = field(default_factory=list)
     peak_resident_bytes: int = 0
+    admit_s: float = 0.0           # job creation (tables, device buffers, table check)
+    round_s: float = 0.0           # batcher rounds (device phases + the round's host work)
+    harvest_s: float = 0.0         # verification and release of finished jobs
+    idle_s: float = 0.0            # waiting for arrivals with nothing resident
 
     def pct(self, q: float) -> float:
         if not self.latencies:
@@ -93,18 +97,23 @@ class Server:
         self.t0 = time.perf_counter()
         self.waiting.sort(key=lambda r: r.arrival)
         while self.waiting or self.active:
+            t = self._now()
             self._admit()
+            now = self._now()
+            st.admit_s += now - t
             st.peak_resident_bytes = max(st.peak_resident_bytes, self.resident)
             if not self.active:
                 nxt = self.waiting[0].arrival
-                time.sleep(max(0.0, min(0.05, nxt - self._now())))
+                time.sleep(max(0.0, min(0.05, nxt - now)))
+                st.idle_s += self._now() - now
                 continue
             if self.round_updates:
                 finished = self.batcher.round_work(self.round_updates)
             else:
                 finished = self.batcher.round(self.max_steps)
             st.rounds += 1
-            now = self._now()
+            t, now = now, self._now()
+            st.round_s += now - t
             for job in finished:
                 r, j = self.active.pop(job.handle.value)
                 r.result = job_result(j, self.verify_threads)
@@ -117,6 +126,7 @@ class Server:
                 if self.progress:
                     self.progress(f"served job {r.rid} ({st.jobs} done) at {now:.1f} s, "
                                   f"latency {r.finished - r.arrival:.2f} s")
+            st.harvest_s += self._now() - now
         st.wall_s = self._now()
         self.batcher.destroy()
         return st

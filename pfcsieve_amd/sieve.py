@@ -39,9 +39,9 @@ class Device:
         return {"name": name.value.decode(), "compute_units": cus.value, "hbm_bytes": hbm.value}
 
     def set_tuning(self, primes_per_lane=0, block_size=0, walk_chunk=0, seg_candidates=0, use_graph=0,
-                   skip_table_check=0, lds_segment=0, walk_forms_off=0):
+                   skip_table_check=0, lds_segment=0, walk_forms_off=0, batch_walk_blocks=0):
         t = Tuning(primes_per_lane, block_size, walk_chunk, seg_candidates, use_graph, skip_table_check,
-                   lds_segment, walk_forms_off)
+                   lds_segment, walk_forms_off, batch_walk_blocks)
         check(lib().pfc_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:

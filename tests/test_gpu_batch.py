@@ -67,6 +67,26 @@ def test_batch_wider_lanes(orc, dev, ppl):
         assert (g.checksum, g.primecount, g.factors) == (cs, pc, fac), (ppl, job)
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+@pytest.mark.parametrize("blocks,ppl", [(3, 1), (64, 1), (5, 2)])
+def test_batch_multi_block_walk_items(orc, dev, graph, blocks, ppl):
+    """Walk items of several blocks of 64 * K candidates (tuning.
+    batch_walk_blocks; short rounds of a serving loop take them by
+    themselves): every job's result is the oracle's, with ragged last items
+    and blocks past a segment's last candidate."""
+    import pfcsieve_amd as PK
+    with pytest.raises(PK.PfcError):
+        dev.set_tuning(batch_walk_blocks=65)
+    dev.set_tuning(batch_walk_blocks=blocks, use_graph=graph, primes_per_lane=ppl)
+    try:
+        got = PK.run_batch(MIX, device=dev, max_steps=300, verify_threads=8)
+    finally:
+        dev.set_tuning()
+    for job, g in zip(MIX, got):
+        cs, pc, fac = _ref(orc, job)
+        assert (g.checksum, g.primecount, g.factors) == (cs, pc, fac), (blocks, ppl, graph, job)
+
+
 def test_batch_multi_segment_jobs(orc, dev):
     import pfcsieve_amd as PK
     dev.set_tuning(seg_candidates=1000)
