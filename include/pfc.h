@@ -187,11 +187,14 @@ int pfc_batcher_pending(const pfc_batcher* b, size_t* n);
  * counted in *ndone, and leave the batcher. */
 int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** done, size_t cap, size_t* ndone);
 
-/* A round sized by work instead of steps: each walking segment takes the
- * step count that gives about max_updates p*n updates over all of them (at
- * least 256 steps), so that rounds, and the wait of a job admitted between
- * them, stay short while large jobs are resident.  No counterpart in the
- * reference (serving). */
+/* A round sized by work instead of steps: about max_updates p*n updates over
+ * all walking segments, an equal share per segment; a segment that needs less
+ * than its share walks everything it has left (up to what one wavefront walks
+ * in such a round), the others floor(share / its candidates) steps, at least
+ * 256.  Rounds, and the wait of a job admitted between them, stay short while
+ * large jobs are resident, and a job of few candidates finishes in a round or
+ * a few whatever else is resident.  No counterpart in the reference
+ * (serving). */
 int pfc_batcher_round_work(pfc_batcher* b, uint64_t max_updates, pfc_job** done, size_t cap,
                            size_t* ndone);
 int pfc_batcher_destroy(pfc_batcher* b);

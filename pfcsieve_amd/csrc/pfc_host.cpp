@@ -22,6 +22,7 @@
 
 #include "../../include/pfc.h"
 #include "pfc_internal.h"
+#include "pfc_share.h"
 
 using pfc::set_error;
 using pfc::hip_fail;
@@ -1519,8 +1520,9 @@ static int batcher_graph_round(pfc_batcher* b, const std::vector<pfc_batch_job>&
     return PFC_OK;
 }
 
-// One round; max_updates != 0 replaces max_steps by the step count that gives
-// about max_updates p*n updates over all walking segments (F|C counted twice).
+// One round; max_updates != 0 replaces max_steps by per-segment step counts
+// that give about max_updates p*n updates over all walking segments (F|C
+// counted twice), an equal share of them per segment (pfc::share_steps).
 static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_updates, pfc_job** done,
                          size_t cap, size_t* ndone) {
     pfc::g_err[0] = 0;
@@ -1575,12 +1577,31 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
             s.stage = 1;
         }
     }
+    uint32_t K, forced_blocks;
+    {
+        std::lock_guard<std::mutex> g(b->dev->mu);
+        K = b->dev->tuning.primes_per_lane;
+        forced_blocks = b->dev->tuning.batch_walk_blocks;
+    }
+    const uint64_t waves = (uint64_t)b->dev->prop.multiProcessorCount * 32;
+    std::vector<uint32_t> slot_steps(ns, max_steps);
     if (max_updates) {
-        uint64_t cand = 0;
-        for (const pfc_batcher::Slot& s : b->slots)
-            if (s.stage == 1) cand += (uint64_t)s.ncand * (s.job->d_rb ? 2 : 1);
-        const uint64_t steps = cand ? max_updates / cand : 0xFFFFFFFFull;
-        max_steps = (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, std::max<uint64_t>(steps, kMinRoundSteps));
+        // A segment walks at most what one wavefront walks in such a round
+        // with every wavefront busy, so that no item outlasts the round.
+        const uint64_t wave_steps = std::max<uint64_t>(kMinRoundSteps, max_updates / (waves * 64 * K));
+        std::vector<uint64_t> cost, cap, steps;
+        std::vector<size_t> at;
+        for (size_t i = 0; i < ns; ++i) {
+            const pfc_batcher::Slot& s = b->slots[i];
+            if (s.stage != 1) continue;
+            at.push_back(i);
+            cost.push_back((uint64_t)s.ncand * (s.job->d_rb ? 2 : 1));
+            cap.push_back(std::min<uint64_t>(s.job->jend - s.jcur, wave_steps));
+        }
+        steps.resize(at.size());
+        pfc::share_steps(cost.data(), cap.data(), at.size(), max_updates, kMinRoundSteps, steps.data());
+        for (size_t k = 0; k < at.size(); ++k)
+            slot_steps[at[k]] = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(steps[k], 0xFFFFFFFFull));
     }
     // 2. work items: one range of items per segment (64*K candidates each),
     // ranges ordered by their items' cost, longest first, so that the queue
@@ -1588,12 +1609,6 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
     std::vector<pfc_batch_job> jobs(ns);
     struct Tmp { pfc_batch_range r; uint32_t count; double cost, setup_cost; uint32_t ncand; };
     std::vector<Tmp> ranges;
-    uint32_t K, forced_blocks;
-    {
-        std::lock_guard<std::mutex> g(b->dev->mu);
-        K = b->dev->tuning.primes_per_lane;
-        forced_blocks = b->dev->tuning.batch_walk_blocks;
-    }
     std::vector<uint8_t> finishing(ns, 0);
     uint64_t nitems = 0;
     // Walk items grow to several blocks only while the round keeps at least
@@ -1601,7 +1616,6 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
     uint64_t walk_blocks = 0;
     for (const pfc_batcher::Slot& s : b->slots)
         if (s.stage == 1) walk_blocks += (s.ncand + 64ull * K - 1) / (64ull * K);
-    const uint64_t waves = (uint64_t)b->dev->prop.multiProcessorCount * 32;
     const uint32_t blocks_cap =
         (uint32_t)std::min<uint64_t>(pfc::kWalkMaxBlocks, std::max<uint64_t>(1, walk_blocks / (4 * waves)));
     for (size_t i = 0; i < ns; ++i) {
@@ -1610,7 +1624,7 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
         if (s.stage != 1) continue;
         pfc_job* j = s.job;
         const uint32_t j0 = s.jcur;
-        const uint32_t j1 = (uint32_t)std::min<uint64_t>(j->jend, (uint64_t)j0 + max_steps);
+        const uint32_t j1 = (uint32_t)std::min<uint64_t>(j->jend, (uint64_t)j0 + slot_steps[i]);
         const uint32_t jw = launch_end(j, j0, j1);          // == j1 unless the test hook is set
         uint32_t flags = (j0 == j->jbegin ? 1u : 0u) | (j1 == j->jend ? 2u : 0u);
         if (s.ncand == 0) {

@@ -3,6 +3,7 @@
 #include <stddef.h>
 #include <stdint.h>
 #include "../../pfcsieve_amd/csrc/pfc_arith.h"
+#include "../../pfcsieve_amd/csrc/pfc_share.h"
 
 extern "C" {
 void t_modulus(uint64_t p, uint32_t out[4]) {
@@ -135,5 +136,10 @@ size_t t_generate(uint64_t lo, uint64_t hi, uint64_t* out, size_t cap) {
         }
     }
     return n;
+}
+// A work-sized batcher round's division of its updates (pfc::share_steps).
+void t_share_steps(const uint64_t* cost, const uint64_t* cap, size_t n, uint64_t budget, uint64_t min_steps,
+                   uint64_t* steps) {
+    pfc::share_steps(cost, cap, n, budget, min_steps, steps);
 }
 }

@@ -177,8 +177,8 @@ def test_setup_table_digit_width_saves_products(L, kind, N, ratio):
 def A():
     out = os.path.join(ROOT, "tests", "host", "_build", "libarith.so")
     src = os.path.join(ROOT, "tests", "host", "arith_shim.cpp")
-    hdr = os.path.join(ROOT, "pfcsieve_amd", "csrc", "pfc_arith.h")
-    if not os.path.exists(out) or os.path.getmtime(out) < max(os.path.getmtime(src), os.path.getmtime(hdr)):
+    hdrs = [os.path.join(ROOT, "pfcsieve_amd", "csrc", h) for h in ("pfc_arith.h", "pfc_share.h")]
+    if not os.path.exists(out) or os.path.getmtime(out) < max(os.path.getmtime(f) for f in [src] + hdrs):
         os.makedirs(os.path.dirname(out), exist_ok=True)
         # clang, as on the device (the same builtins are compiled)
         cxx = "/opt/rocm/llvm/bin/clang++" if os.path.exists("/opt/rocm/llvm/bin/clang++") else "g++"
@@ -216,6 +216,7 @@ def A():
     L.t_medium_fits.argtypes = [u64, u32]
     L.t_medium_walk.argtypes = [u64, u64, C.POINTER(u32), u32]
     L.t_medium_walk.restype = u64
+    L.t_share_steps.argtypes = [C.POINTER(u64), C.POINTER(u64), C.c_size_t, u64, u64, C.POINTER(u64)]
     return L
 
 
@@ -586,3 +587,73 @@ def test_start_residue_against_bigint(L, A, kind, N):
     for p in _primes_mix(rng, 5):
         s = 64 - p.bit_length()
         assert A.t_start_residue_split(ch, gr, t.npos, t.digit_bits, p, 256) == (X % p) << s
+
+
+# ----------------------------------------------------------------- round shares
+
+
+def _share(A, cost, cap, budget, min_steps):
+    n = len(cost)
+    arr = C.c_uint64 * max(n, 1)
+    out = arr()
+    A.t_share_steps(arr(*cost), arr(*cap), n, budget, min_steps, out)
+    return list(out)[:n]
+
+
+def _water_level(demand, budget):
+    """Largest integer L with sum(min(d, L)) <= budget (None: every demand fits)."""
+    if sum(demand) <= budget:
+        return None
+    lo, hi = 0, max(demand)
+    while lo < hi:
+        mid = (lo + hi + 1) // 2
+        if sum(min(d, mid) for d in demand) <= budget:
+            lo = mid
+        else:
+            hi = mid - 1
+    return lo
+
+
+def test_round_share_is_water_filling(A):
+    """A work-sized round gives every segment the same share of its updates
+    and a segment that needs less its whole remainder: the level is the
+    largest L with sum(min(cost * cap, L)) <= budget, found here by
+    bisection, and a segment below it takes cap steps, the others
+    floor(L / cost) (at least min(min_steps, cap))."""
+    rng = random.Random(11)
+    cases = [([], [], 10, 256),
+             ([10**6], [5000], 4 * 10**11, 256),          # alone: budget / cost steps
+             ([10**4, 10**8], [10**6, 10**6], 4 * 10**11, 256),
+             ([0, 7, 10**9], [300, 10**6, 10**6], 10**9, 256),
+             ([1, 1, 1], [10, 10, 10], 10**6, 256)]
+    for _ in range(300):
+        n = rng.randrange(1, 40)
+        cost = [rng.choice([0, rng.randrange(1, 100), rng.randrange(1, 10**9)]) for _ in range(n)]
+        cap = [rng.randrange(1, 10**6) for _ in range(n)]
+        cases.append((cost, cap, rng.randrange(1, 10**13), rng.choice([1, 256])))
+    for cost, cap, budget, m0 in cases:
+        got = _share(A, cost, cap, budget, m0)
+        demand = [c * k for c, k in zip(cost, cap)]
+        L = _water_level(demand, budget)
+        want = []
+        for c, k, d in zip(cost, cap, demand):
+            if L is None or d <= L:
+                want.append(k)
+            else:
+                want.append(min(k, max(L // c, min(m0, k))))
+        assert got == want, (cost, cap, budget, m0)
+        # the round's updates stay within the budget, apart from the minimum steps
+        extra = sum(c * max(0, min(m0, k) - (L // c if c and L is not None else k))
+                    for c, k in zip(cost, cap))
+        assert sum(c * s for c, s in zip(cost, got)) <= budget + extra
+
+
+def test_round_share_small_segment_finishes_beside_large_ones(A):
+    """A segment of 10^4 candidates beside three of 10^8 gets its whole
+    remainder in one round of 4e11 updates; the large ones split the rest
+    evenly (the same step count, equal costs)."""
+    cost = [10**4, 10**8, 10**8, 10**8]
+    cap = [760000] * 4
+    got = _share(A, cost, cap, 4 * 10**11, 256)
+    assert got[0] == 760000
+    assert got[1] == got[2] == got[3] == (4 * 10**11 - 10**4 * 760000) // 3 // 10**8
