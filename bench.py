@@ -316,7 +316,7 @@ def stream_bound(units, rates):
 
 
 def serve_prefix(dev, njobs, rates, round_updates, progress, start=0, full_rate=False, latencies_path=None,
-                 rate=0.0):
+                 rate=0.0, overlap=True):
     """Jobs [start, start + njobs) of the seeded 10,000-job stream through the
     serving loop.  Poisson arrivals at 0.8 x the capacity that a device-work
     bound gives (stream_bound at the measured kernel rates), instead of a
@@ -326,7 +326,9 @@ def serve_prefix(dev, njobs, rates, round_updates, progress, start=0, full_rate=
     stream's own arrival times, shifted so that its first job arrives one
     inter-arrival gap after the clock starts; it starts with nothing
     resident.  latencies_path: write every job's (index, arrival, latency)
-    as JSON there.  rate > 0 fixes the arrival rate (jobs/s) instead."""
+    as JSON there.  rate > 0 fixes the arrival rate (jobs/s) instead.
+    overlap=False harvests finished jobs between rounds instead of on worker
+    threads beside them."""
     import pfcsieve_amd.serve as S
     from pfcsieve_amd import workload as Wl
     stream = Wl.mix(10_000, seed=2024)
@@ -341,7 +343,7 @@ def serve_prefix(dev, njobs, rates, round_updates, progress, start=0, full_rate=
     info = dev.info()
     say = (lambda m: _progress(True, m)) if progress else None
     srv = S.Server(dev, hbm_budget=int(info["hbm_bytes"] * 0.8), verify_threads=16, progress=say,
-                   round_updates=round_updates)
+                   round_updates=round_updates, overlap=overlap)
     for k, (u, a) in enumerate(zip(units, arrivals)):
         srv.submit(S.Request(start + k, u.desc(), a))
     st = srv.run()
@@ -359,7 +361,7 @@ def serve_prefix(dev, njobs, rates, round_updates, progress, start=0, full_rate=
             "jobs_per_s": st.jobs / st.wall_s, "p50_latency_s": st.pct(50), "p99_latency_s": st.pct(99),
             "max_latency_s": max(st.latencies) if st.latencies else None,
             "host_split_s": {"admit": st.admit_s, "rounds": st.round_s, "harvest": st.harvest_s,
-                             "idle": st.idle_s},
+                             "idle": st.idle_s, "harvest_beside_rounds": st.overlap},
             "peak_resident_bytes": st.peak_resident_bytes, "round_policy": f"about {round_updates:.3g} p*n "
                                                                            f"updates per round"}
 
@@ -540,6 +542,8 @@ def main():
                     help="arrival rate from the whole 10,000-job stream's device-work bound (slices)")
     ap.add_argument("--serve-latencies", default="", help="write every served job's latency to this JSON file")
     ap.add_argument("--serve-rate", type=float, default=0.0, help="fixed arrival rate, jobs/s (0: from the bound)")
+    ap.add_argument("--serve-serial", action="store_true",
+                    help="harvest served jobs between rounds, not on worker threads beside them")
     ap.add_argument("--serve-round-updates", type=float, default=4e11)
     ap.add_argument("--no-headline", action="store_true")
     ap.add_argument("--kernel-rates", action="store_true", help="also time k_generate and k_setup standalone")
@@ -614,7 +618,8 @@ def main():
                  "generate": kr["generate_p1e12_2e9"]["candidates_per_s"]}
         line["served"] = serve_prefix(dev, args.serve_jobs, rates, int(args.serve_round_updates), args.progress,
                                       start=args.serve_from, full_rate=args.serve_full_rate,
-                                      latencies_path=args.serve_latencies or None, rate=args.serve_rate)
+                                      latencies_path=args.serve_latencies or None, rate=args.serve_rate,
+                                      overlap=not args.serve_serial)
         line["served"]["rates"] = rates
     line["config"]["small_job"] = small_job(dev)
     if args.kernel_rates and "kernel_rates" not in line:

@@ -18,12 +18,15 @@ REQS = [
 ]
 
 
+@pytest.mark.parametrize("overlap", [True, False])
 @pytest.mark.parametrize("round_updates", [0, 3 * 10**6])
-def test_served_results_equal_standalone(orc, dev, round_updates):
+def test_served_results_equal_standalone(orc, dev, round_updates, overlap):
     """Rounds of at most 400 steps per segment, or of about 3e6 updates
-    (pfc_batcher_round_work: the step count follows the resident work)."""
+    (pfc_batcher_round_work: an equal share per segment); finished jobs
+    harvested on worker threads beside the rounds, or between them."""
     import pfcsieve_amd.serve as S
-    srv = S.Server(dev, hbm_budget=1 << 40, max_steps=400, verify_threads=8, round_updates=round_updates)
+    srv = S.Server(dev, hbm_budget=1 << 40, max_steps=400, verify_threads=8, round_updates=round_updates,
+                   overlap=overlap)
     for k, (d, t) in enumerate(REQS):
         srv.submit(S.Request(k, d, t))
     st = srv.run()
