@@ -35,7 +35,8 @@ Workloads (synthetic: every candidate prime is generated on the device):
   units drawn as in the mix), Poisson arrivals at 0.8x the capacity that the
   prefix's own device-work bound gives at the kernels' measured rates (walk,
   set-up, generation), admitted by HBM footprint, rounds of about 4e11 p*n
-  updates; throughput and p50/p99 job latency.
+  updates shared equally among the resident jobs, finished jobs harvested
+  beside the next rounds; throughput and p50/p99 job latency.
 * ``cpu_baseline`` -- the CPU oracle (OpenMP over candidates) on a bounded
   p-sample of the headline job, plus the reference's CPU-runnable case in
   full.
