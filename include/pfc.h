@@ -170,7 +170,9 @@ int pfc_job_debug_shorten_walk(pfc_job* job, uint32_t steps);
 /* No counterpart in the reference, which runs one p-range per launch
  * sequence.  A batcher runs many jobs together: each round generates new
  * p-segments for jobs that need one, cuts every active segment into work
- * items (64*K candidates x up to max_steps multiplier steps, longest first)
+ * items (blocks of 64*K candidates x up to max_steps multiplier steps,
+ * longest first; a walk item covers more blocks when the round's steps are
+ * few, pfc_tuning.batch_walk_blocks)
  * and drains them with persistent kernels fed by device work queues: one
  * launch for the set-up of new segments, one per walk mode for the walk, one
  * for the checksum fold of finished segments.  Job state stays resident on

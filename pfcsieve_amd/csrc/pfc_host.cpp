@@ -1251,8 +1251,9 @@ extern "C" int pfc_autotune(pfc_device* dev, const pfc_job_desc* desc, pfc_tunin
 //   1. jobs that need a new p-segment get one (clear control block, generate
 //      candidates) -- all jobs' generation launches are queued back to back;
 //   2. the candidates of every active segment are cut into work items of
-//      64*K candidates x up to `max_steps` multiplier steps, described by one
-//      range per segment and ordered by decreasing cost (longest first), and
+//      blocks of 64*K candidates x up to `max_steps` multiplier steps (or a
+//      segment's share of a round sized by work), described by one range per
+//      segment and ordered by decreasing cost (longest first), and
 //      persistent kernels drain them from device queues (a counter each;
 //      an item finds its segment by binary search): the set-up of segments
 //      that start in this round, the walk of each walk mode, the checksum
