@@ -1604,9 +1604,9 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
         for (size_t k = 0; k < at.size(); ++k)
             slot_steps[at[k]] = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(steps[k], 0xFFFFFFFFull));
     }
-    // 2. work items: one range of items per segment (64*K candidates each),
-    // ranges ordered by their items' cost, longest first, so that the queue
-    // ends on short items.
+    // 2. work items: one range of items per segment (blocks of 64*K
+    // candidates each), ranges ordered by their items' cost, longest first,
+    // so that the queue ends on short items.
     std::vector<pfc_batch_job> jobs(ns);
     struct Tmp { pfc_batch_range r; uint32_t count; double cost, setup_cost; uint32_t ncand; };
     std::vector<Tmp> ranges;

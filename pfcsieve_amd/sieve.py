@@ -325,7 +325,8 @@ class Batcher:
         return [self._jobs.pop(arr[i]) for i in range(min(nd.value, cap))]
 
     def round_work(self, max_updates: int) -> List[Job]:
-        """One round of about max_updates p*n updates (pfc_batcher_round_work)."""
+        """One round of about max_updates p*n updates, an equal share per
+        resident segment (pfc_batcher_round_work)."""
         cap = max(len(self._jobs), 1)
         arr = (C.c_void_p * cap)()
         nd = C.c_size_t()
