@@ -1587,16 +1587,24 @@ static int batcher_round(pfc_batcher* b, uint32_t max_steps, uint64_t max_update
     const uint64_t waves = (uint64_t)b->dev->prop.multiProcessorCount * 32;
     std::vector<uint32_t> slot_steps(ns, max_steps);
     if (max_updates) {
-        // A segment walks at most what one wavefront walks in such a round
-        // with every wavefront busy, so that no item outlasts the round.
-        const uint64_t wave_steps = std::max<uint64_t>(kMinRoundSteps, max_updates / (waves * 64 * K));
         std::vector<uint64_t> cost, cap, steps;
         std::vector<size_t> at;
+        uint64_t total = 0;
         for (size_t i = 0; i < ns; ++i) {
             const pfc_batcher::Slot& s = b->slots[i];
             if (s.stage != 1) continue;
             at.push_back(i);
             cost.push_back((uint64_t)s.ncand * (s.job->d_rb ? 2 : 1));
+            total += cost.back();
+        }
+        // A segment walks at most what one wavefront walks in such a round
+        // with every wavefront busy (so that no item outlasts the round), or
+        // the step count all segments would take if all took the same, if
+        // that is more.
+        const uint64_t wave_steps = std::max<uint64_t>(
+            std::max<uint64_t>(kMinRoundSteps, max_updates / (waves * 64 * K)), total ? max_updates / total : 0);
+        for (size_t k = 0; k < at.size(); ++k) {
+            const pfc_batcher::Slot& s = b->slots[at[k]];
             cap.push_back(std::min<uint64_t>(s.job->jend - s.jcur, wave_steps));
         }
         steps.resize(at.size());

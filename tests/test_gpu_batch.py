@@ -225,3 +225,36 @@ def test_graph_captured_rounds_match_eager_and_oracle(orc, dev, ppl):
         assert (g.checksum, g.primecount, g.factors) == (e.checksum, e.primecount, e.factors), d
         cs, pc, fac = _ref(orc, d)
         assert (g.checksum, g.primecount, g.factors) == (cs, pc, fac), d
+
+
+def test_round_work_small_job_finishes_beside_a_large_one(orc, dev):
+    """pfc_batcher_round_work shares a round's updates equally among the
+    segments: beside a job of ~1.45e6 candidates that needs three rounds of
+    1.2e10 updates, a job of ~360 candidates takes all of its 19,899 steps in
+    the first round (dividing the round by candidates would have given it
+    ~8,300 steps a round, three rounds).  Both results are the oracle's."""
+    import pfcsieve_amd as PK
+    from pfcsieve_amd.sieve import Batcher, Job, job_result
+    big = (F, 10**9, 10**9 + 3 * 10**7, 101, 20000)
+    small = (F, 10**12, 10**12 + 10**4, 101, 20000)
+    jb, js = Job(dev, *big), Job(dev, *small)
+    b = Batcher(dev)
+    try:
+        b.add(jb)
+        b.add(js)
+        first = b.round_work(12 * 10**9)
+        assert [j.handle.value for j in first] == [js.handle.value]
+        rounds = 1
+        while not b.round_work(12 * 10**9):
+            rounds += 1
+            assert rounds < 10
+        assert rounds + 1 >= 3
+        for job, j in ((small, js), (big, jb)):
+            r = job_result(j, 8)
+            ref = orc.run(*job)
+            assert (r.checksum, r.primecount, r.factors) == \
+                (ref.checksum, ref.primecount, [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in ref.factors]), job
+    finally:
+        b.destroy()
+        jb.destroy()
+        js.destroy()
