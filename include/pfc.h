@@ -192,8 +192,9 @@ int pfc_batcher_round(pfc_batcher* b, uint32_t max_steps, pfc_job** done, size_t
 /* A round sized by work instead of steps: about max_updates p*n updates over
  * all walking segments, an equal share per segment; a segment that needs less
  * than its share walks everything it has left (up to what one wavefront walks
- * in such a round), the others floor(share / its candidates) steps, at least
- * 256.  Rounds, and the wait of a job admitted between them, stay short while
+ * in such a round, or the step count every segment would get if all got the
+ * same, whichever is more), the others floor(share / its candidates) steps,
+ * at least 256.  Rounds, and the wait of a job admitted between them, stay short while
  * large jobs are resident, and a job of few candidates finishes in a round or
  * a few whatever else is resident.  No counterpart in the reference
  * (serving). */
