@@ -68,7 +68,7 @@ def test_batch_wider_lanes(orc, dev, ppl):
 
 
 @pytest.mark.parametrize("graph", [0, 1])
-@pytest.mark.parametrize("blocks,ppl", [(3, 1), (64, 1), (5, 2)])
+@pytest.mark.parametrize("blocks,ppl", [(3, 1), (64, 1), (5, 2), (7, 4)])
 def test_batch_multi_block_walk_items(orc, dev, graph, blocks, ppl):
     """Walk items of several blocks of 64 * K candidates (tuning.
     batch_walk_blocks; short rounds of a serving loop take them by
