@@ -81,8 +81,8 @@ def prime_flags(n):
 
 def narrow_fits(p, bmax):
     """The walk's narrow form (plain residues below 2p, five multiplies per
-    step, pfc::narrow_fits): 2^34 < p < 2^55, 1 <= bmax < 2^22, p * bmax < 2^61."""
-    return 2**34 < p < 2**55 and 1 <= bmax < 2**22 and p * bmax < 2**61
+    step, pfc::narrow_fits): 2^34 < p < 2^55, 1 <= bmax < 2^22, p * bmax < 2^62."""
+    return 2**34 < p < 2**55 and 1 <= bmax < 2**22 and p * bmax < 2**62
 
 
 def walk_muls(nmin, nmax, chunk, pmax):
@@ -184,7 +184,7 @@ def roofline(head, mul_rates):
         "mul_per_update": muls / upd, "narrow_share": narrow / upd, "updates_per_launch": upd * cand / launches,
         "avg_launch_ms": avg_s * 1e3,
         "frac_of_measured_mad_rate": achieved / mad_rate,
-        "issue": "the walk is bound by VALU issue: in the narrow form (p * n < 2^61) each update is five "
+        "issue": "the walk is bound by VALU issue: in the narrow form (p * n < 2^62) each update is five "
                  "32-bit multiply instructions, and every pair of updates adds a compare and a masked add "
                  "(the reduction below p) and a 64-bit add and compare (the band test); rocprof VALU and "
                  "SALU counts per update in profiles/",
@@ -295,7 +295,7 @@ def narrow_steps(u):
     pmax = u.pmax - 1
     if not (2**34 < u.pmin and pmax < 2**55):
         return 0
-    nlim = min(2**22 - 1, (2**61 - 1) // pmax)
+    nlim = min(2**22 - 1, (2**62 - 1) // pmax)
     hi = min(u.nmax, nlim + 1)
     return Wl.walk_steps(Wl.Unit(u.modes, u.pmin, u.pmax, u.nmin, hi)) if hi > u.nmin else 0
 
@@ -398,14 +398,14 @@ def kernel_rates(dev, reps=3):
                      "integers_per_s": span / best}
         d_p.free()
         ctrl.free()
-    # The medium walk (p * n above 2^61, p below 2^63): -! -c, n in [9e5, 1e6),
+    # The medium walk (p * n above 2^62, p below 2^63): -! -c, n in [9e5, 1e6),
     # p from 2^61.
     j = P.Job(dev, 5, 2**61, 2**61 + 5 * 10**8, 900_000, 10**6)
     j.run()
     s = j.stats()
     j.destroy()
     out["walk_medium_p2e61"] = {"workload": "-! -c n [9e5, 1e6) p in [2^61, 2^61 + 5e8): the medium walk "
-                                            "(p * n > 2^61, p < 2^63), walk launches timed by HIP events",
+                                            "(p * n > 2^62, p < 2^63), walk launches timed by HIP events",
                                  "candidates": s.primecount, "walk_updates": s.walk_updates,
                                  "walk_ms": s.walk_ms, "walk_updates_per_s": s.walk_updates / (s.walk_ms * 1e-3)}
     products = P.setup_table_cost(0, 10**6 - 1)

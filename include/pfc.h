@@ -83,7 +83,7 @@ typedef struct pfc_tuning {
     uint32_t lds_segment;      /* integers sieved per generator workgroup and staged in LDS:
                                   15360 (default), 30720 or 61440 */
     uint32_t walk_forms_off;   /* walk arithmetic forms not to take (for comparisons): bit 0 the
-                                  narrow form (plain residues, 2^34 < p < 2^55, p * n < 2^61), bit 1
+                                  narrow form (plain residues, 2^34 < p < 2^55, p * n < 2^62), bit 1
                                   the medium form (2^32 < p < 2^63, n < 2^28); 0 (default): each
                                   launch takes the cheapest form its primes and multipliers allow */
     uint32_t batch_walk_blocks; /* blocks of 64 * primes_per_lane candidates per batcher walk item,
@@ -312,7 +312,7 @@ int pfc_stage_setup(const uint64_t* d_p, const pfc_ctrl* d_ctrl, uint32_t* d_v, 
  * largest multiplier j1 - 1, F, C and F|C steps are tested in pairs, and a
  * pair with both multipliers at most 65,536 is one step by their product;
  * such a wavefront walks plain residues below 2p when 2^34 < p < 2^55 and
- * p * (j1 - 1) < 2^61, or half-scale residues when p < 2^63, converting on
+ * p * (j1 - 1) < 2^62, or half-scale residues when p < 2^63, converting on
  * entry and returning them canonical.  P steps are tested one by one here,
  * in the shifted form, since the call cannot bound d_list's values.  The
  * records are the same either way.) */

@@ -140,27 +140,28 @@ PFC_HD uint32_t walk_k0(uint32_t d0) { return 0u - d0; }
 // ---------------------------------------------------------------- narrow walk
 
 // Walk arithmetic for moduli with 2^34 < p < 2^55 and multipliers n with
-// p * n < 2^61 and n < 2^22 ("narrow" moduli; the headline's p ~ 10^12 with n
+// p * n < 2^62 and n < 2^22 ("narrow" moduli; the headline's p ~ 10^12 with n
 // below 10^6 is one).  Residues are plain integers X in [0, 2p), not reduced
-// and not shifted.  Then t = X * n < 2^62 is one 64-bit word and a quotient
+// and not shifted.  Then t = X * n < 2^63 is one 64-bit word and a quotient
 // estimate from its high word alone leaves a remainder below 2p, so a step is
 // five multiplies and no correction:
-//   t  = X0 * n + ((X1 * n) << 32)     X1 * n < 2^30 (one multiply, one
+//   t  = X0 * n + ((X1 * n) << 32)     X1 * n < 2^31 (one multiply, one
 //                                      multiply-add on the pair <X1*n, 0>)
-//   q  = hi32(t1 * m)                  m = floor(2^64 / p) (or one less)
+//   q  = hi32(t1 * m)                  m = floor(2^64 / p)
 //   r  = t - q * p = q * np0 + <t1 - q * (p1 + 1), t0>   (mod 2^64)
 // with t1, t0 the words of t, p1 the high word of p and np0 = 2^32 - p0
 // (p0 != 0: p is odd).  The correction term is exact: with
-// 2^64 / p = m + phi, 0 <= phi < 2 (m may be one below the floor),
+// 2^64 / p = m + phi, 0 <= phi < 1 (m is the floor, from the exact 3-by-2
+// reciprocal),
 //   t / p - q < 1 + t1 * phi / 2^32 + t0 / p < 1 + 2^-1 + 2^-2,
-// because t1 < 2^30 and t0 < 2^32 < p / 4; so floor(t / p) <= q + 1 and
-// 0 <= r < 2p.  q < 2n < 2^23 and p1 + 1 <= 2^23, so q * (p1 + 1) fits the
+// because t1 < 2^31 and t0 < 2^32 < p / 4; so floor(t / p) <= q + 1 and
+// 0 <= r < 2p (q <= t / p since m <= 2^64 / p).  q < 2n < 2^23 and p1 + 1 <= 2^23, so q * (p1 + 1) fits the
 // signed 24-bit multiply-add (v_mad_i32_i24) on the device.
 PFC_HD bool narrow_fits(uint64_t p, uint32_t bmax) {
     if (bmax == 0 || bmax >= (1u << 22) || p <= (1ull << 34) || p >= (1ull << 55)) return false;
     const uint64_t a = (p >> 32) * bmax;                       // < 2^45
-    if (a >= (1ull << 29)) return false;
-    return (a << 32) + (uint64_t)(uint32_t)p * bmax < (1ull << 61);
+    if (a >= (1ull << 30)) return false;
+    return (a << 32) + (uint64_t)(uint32_t)p * bmax < (1ull << 62);
 }
 
 // floor(2^64 / p) from the 3-by-2 reciprocal of d = p << s (s < 30 here):

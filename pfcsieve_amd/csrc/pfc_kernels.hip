@@ -397,7 +397,7 @@ struct WalkArgs {
 // passes a fraction below 2^-13 of the residues.
 //
 // Narrow form.  A wavefront whose every p lies in (2^34, 2^55) with
-// p * bmax < 2^61 (and bmax < 2^22) walks the launch on plain residues
+// p * bmax < 2^62 (and bmax < 2^22) walks the launch on plain residues
 // X = x >> s in [0, 2p) with pfc::narrow_mul: five multiply instructions and
 // no correction per step, against ten instructions for the shifted step.
 // Groups and bands are the same, unshifted; a tested step first reduces its

@@ -709,6 +709,36 @@ def test_walk_forms_find_planted_factors(dev, planted_narrow, forms_off, ppl):
         dev.set_tuning()
 
 
+NARROW_TOP = [
+    (F | Cm, 3 * 10**12, 3 * 10**12 + 2 * 10**5, 900_000, 1_300_000),        # p * n up to 2^61.8
+    (F, 4_398_000_000_000, 4_398_000_000_000 + 2 * 10**5, 950_000, 1_060_000),  # crosses 2^62 at n ~ 1.0486e6
+    (P, 2**41, 2**41 + 4 * 10**5, 101, 2_000_000),                          # primes up to 2e6: p * q < 2^62
+]
+
+
+@pytest.mark.parametrize("forms_off", [0, 1])
+def test_narrow_form_near_its_bound(orc, dev, forms_off):
+    """Moduli and multipliers with p * n in [2^61, 2^62), the top of the narrow
+    form's range (t = X * n up to 2^63), and a job whose n-range crosses
+    2^62 / p so that its launches change form: the oracle's results with the
+    narrow form on (0) and off (1, the medium form), alone and batched."""
+    import pfcsieve_amd as PK
+    refs = []
+    for desc in NARROW_TOP:
+        r = orc.run(*desc)
+        refs.append((r.checksum, r.primecount, [(int(f["p"]), int(f["nc"]), int(f["type"])) for f in r.factors]))
+    dev.set_tuning(walk_forms_off=forms_off, walk_chunk=50_000)
+    try:
+        for desc, ref in zip(NARROW_TOP, refs):
+            got = PK.run_sieve(*desc, device=dev, verify_threads=8)
+            assert (got.checksum, got.primecount, got.factors) == ref, desc
+        for desc, g, ref in zip(NARROW_TOP, PK.run_batch(NARROW_TOP, device=dev, max_steps=40_000,
+                                                          verify_threads=8), refs):
+            assert (g.checksum, g.primecount, g.factors) == ref, desc
+    finally:
+        dev.set_tuning()
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("modes,lo,width,nmin,nmax,cut", [
     (F | Cm, 10**12, 2 * 10**8, 101, 10**6, 73_212_347),            # the -! -c benchmark shape

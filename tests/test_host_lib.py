@@ -348,19 +348,19 @@ def _narrow_moduli(rng, k):
 
 
 def _narrow_bmax(p):
-    """The largest multiplier the narrow walk takes for p: p * n < 2^61, n < 2^22."""
-    return min((2**61 - 1) // p, 2**22 - 1)
+    """The largest multiplier the narrow walk takes for p: p * n < 2^62, n < 2^22."""
+    return min((2**62 - 1) // p, 2**22 - 1)
 
 
 def test_narrow_fits_bounds(A):
     """narrow_fits holds exactly for 2^34 < p < 2^55, 1 <= bmax < 2^22 and
-    p * bmax < 2^61."""
+    p * bmax < 2^62."""
     rng = random.Random(21)
     for p in _narrow_moduli(rng, 300) + [2**34 - 1, 2**34, 2**55, 2**55 + 1, 2**60 + 1]:
-        for b in [0, 1, 2, 2**22 - 1, 2**22, 2**32 - 1, (2**61 - 1) // p, (2**61 - 1) // p + 1,
+        for b in [0, 1, 2, 2**22 - 1, 2**22, 2**32 - 1, (2**62 - 1) // p, (2**62 - 1) // p + 1,
                   rng.randrange(1, 2**22)]:
             b = min(b, 2**32 - 1)
-            want = 2**34 < p < 2**55 and 1 <= b < 2**22 and p * b < 2**61
+            want = 2**34 < p < 2**55 and 1 <= b < 2**22 and p * b < 2**62
             assert A.t_narrow_fits(p, b) == int(want), (p, b)
 
 
@@ -402,7 +402,7 @@ def test_narrow_step_against_bigint(A):
 
 def test_narrow_step_near_multiples(A):
     """Products t = X * n just below and above multiples of p (where a wrong
-    quotient estimate would show), including t close to 2^62."""
+    quotient estimate would show), including t close to 2^63."""
     rng = random.Random(24)
     for p in _narrow_moduli(rng, 800):
         bmax = _narrow_bmax(p)
