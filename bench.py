@@ -557,8 +557,8 @@ def main():
     ap.add_argument("--progress", action="store_true", help="progress lines on stderr")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding over p is a design "
-                                   "only, not built", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the benchmark runs on exactly one GPU; sharding over p across GPUs is "
+                                   "designed but not built", "n_gpus": args.gpus}))
         sys.exit(2)
     K, W = max(1, args.steps), max(0, args.warmup)
 
