@@ -37,3 +37,18 @@ def test_walk_steps():
     assert W.walk_steps(u) == sum(1 for n in range(101, 1101) if W._is_prime_small(n))
     u = W.Unit(4, 10**10, 10**10 + 10**6, 101, 1101)
     assert W.walk_steps(u) == 1000 - sum(1 for n in range(101, 1101) if W._is_prime_small(n))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N with N > 1 exits with status 2 and a JSON line that
+    says the benchmark runs on one GPU (before touching a device)."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
