@@ -426,9 +426,11 @@ def walk_forms(dev):
     """The walk of one p-segment (p-width 1e8) in each arithmetic form its
     moduli allow (pfc_tuning.walk_forms_off): -! -c at n in [101, 2e5) and
     [9e5, 1e6) from p = 1e12 (narrow, medium, shifted) and from p = 2^61
-    (medium, shifted), -! at n in [9e5, 1e6) from 2^61, and -# at
-    n in [101, 1e6) from p = 2^62 - 1e8 (medium, shifted); walk launches
-    timed by HIP events, the same candidates and multipliers in every form."""
+    (medium, shifted), -! at n in [9e5, 1e6) from 2^61, -# at n in
+    [101, 1e6) from p = 2^62 - 1e8 (medium, shifted), and -! -c and -! at
+    n in [101, 65536) (every pair a product step in the shifted form) from
+    1e12 and 2^61; walk launches timed by HIP events, the same candidates and
+    multipliers in every form."""
     import pfcsieve_amd as P
     from pfcsieve_amd import workload as Wl
     out = {}
@@ -438,7 +440,10 @@ def walk_forms(dev):
             ("fc_p2e61", 5, 2**61, 101, 200_000, (("medium", 0), ("shifted", 2))),
             ("fc_p2e61_n9e5", 5, 2**61, 900_000, 1_000_000, (("medium", 0), ("shifted", 2))),
             ("f_p2e61_n9e5", 1, 2**61, 900_000, 1_000_000, (("medium", 0), ("shifted", 2))),
-            ("p_p2e62", 2, 2**62 - 10**8, 101, 1_000_000, (("medium", 0), ("shifted", 2)))):
+            ("p_p2e62", 2, 2**62 - 10**8, 101, 1_000_000, (("medium", 0), ("shifted", 2))),
+            ("fc_p1e12_n64k", 5, 10**12, 101, 65_536, (("narrow", 0), ("medium", 1), ("shifted", 3))),
+            ("fc_p2e61_n64k", 5, 2**61, 101, 65_536, (("medium", 0), ("shifted", 2))),
+            ("f_p2e61_n64k", 1, 2**61, 101, 65_536, (("medium", 0), ("shifted", 2)))):
         rows = {}
         for form, off in forms:
             dev.set_tuning(walk_forms_off=off, seg_candidates=1 << 24)
