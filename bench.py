@@ -425,7 +425,9 @@ def kernel_rates(dev, reps=3):
 def walk_forms(dev):
     """The walk of one p-segment (p-width 1e8) in each arithmetic form its
     moduli allow (pfc_tuning.walk_forms_off): -! -c at n in [101, 2e5) and
-    [9e5, 1e6) from p = 1e12 (narrow, medium, shifted) and from p = 2^61
+    [9e5, 1e6) from p = 1e12 and at n in [9e5, 1.3e6) from 3e12 (p * n
+    between 2^61 and 2^62: the top of the narrow range) (narrow, medium,
+    shifted), from p = 2^61
     (medium, shifted), -! at n in [9e5, 1e6) from 2^61, -# at n in
     [101, 1e6) from p = 2^62 - 1e8 (medium, shifted), and -! -c and -! at
     n in [101, 65536) (every pair a product step in the shifted form) from
@@ -437,6 +439,7 @@ def walk_forms(dev):
     for name, modes, pmin, nmin, nmax, forms in (
             ("fc_p1e12", 5, 10**12, 101, 200_000, (("narrow", 0), ("medium", 1), ("shifted", 3))),
             ("fc_p1e12_n9e5", 5, 10**12, 900_000, 1_000_000, (("narrow", 0), ("medium", 1), ("shifted", 3))),
+            ("fc_p3e12_n9e5", 5, 3 * 10**12, 900_000, 1_300_000, (("narrow", 0), ("medium", 1), ("shifted", 3))),
             ("fc_p2e61", 5, 2**61, 101, 200_000, (("medium", 0), ("shifted", 2))),
             ("fc_p2e61_n9e5", 5, 2**61, 900_000, 1_000_000, (("medium", 0), ("shifted", 2))),
             ("f_p2e61_n9e5", 1, 2**61, 900_000, 1_000_000, (("medium", 0), ("shifted", 2))),
